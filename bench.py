@@ -1,0 +1,193 @@
+"""Benchmark: D2Q9 TRT Taylor-Green vortex, periodic 4096 x 4096, on one MI355X.
+
+A step is one time step of the lattice (collide -> stream -> BCs) over the
+whole 4096^2 lattice, i.e. one fused pull collide-stream kernel launch.
+``value`` is the plain FP64 throughput in MLUPS (million lattice updates per
+second) over K timed steps, populations resident in HBM.  The same run with
+FP16-deviation storage (FP32 arithmetic) is reported beside it with its
+relative L2 velocity error against the FP64 run.
+
+``roofline`` prices the fused kernel against MI355X's 8 TB/s: algorithmic
+bytes per launch = 2 * Q * sizeof(population) * nodes (one read and one write
+of every population), divided by the kernel's average duration measured with
+HIP events on the library's stream.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 4096]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "latticeboltzmann.jl_amd"))
+
+METRIC = ("MLUPS + %HBM roofline, D2Q9 TRT 4096²: FP16-store vs FP64 plain at "
+          "err≤1e-4")
+HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(REPO, "profiles", "hbm_traffic.json")
+
+
+def tgv_populations(L, q, n, u0=0.02):
+    c = (np.arange(n) + 0.5) * (2 * np.pi / n)
+    X, Y = np.meshgrid(c, c, indexing="ij")
+    ux = u0 * np.sin(X) * np.cos(Y)
+    uy = -u0 * np.cos(X) * np.sin(Y)
+    rho = 1.0 - (3.0 * u0 * u0 / 4.0) * (np.cos(2 * X) + np.cos(2 * Y))
+    return L.equilibrium_populations(q, rho, ux, uy)
+
+
+def run_storage(L, q, f0, cm, storage, steps, warmup):
+    model = L.LatticeBoltzmannModel.from_populations(q, f0, cm, storage=storage)
+    if warmup:
+        L.simulate(model, range(warmup))
+    L.synchronize()
+    t0 = time.perf_counter()
+    kernel_ms, nlaunch = L.simulate_timed(model, steps)
+    L.synchronize()
+    wall = time.perf_counter() - t0
+    return model, wall, kernel_ms, nlaunch
+
+
+def traffic_for(key):
+    try:
+        with open(TRAFFIC_FILE) as fh:
+            return json.load(fh).get(key)
+    except (OSError, ValueError):
+        return None
+
+
+def cpu_baseline(n, tau_s, tau_a, f0, target_s):
+    """The FP64 CPU restatement (the oracle) timed on this host's cores on a
+    bounded sample of the same workload."""
+    from oracle import clib
+    threads = min(os.cpu_count() or 1, 16)
+    clib.set_threads(threads)
+    S = np.asfortranarray(f0.copy())
+    C = np.asfortranarray(f0.copy())
+    t0 = time.perf_counter()
+    clib.simulate(9, S, C, 1, [tau_s, tau_a], 1)
+    t1 = time.perf_counter() - t0
+    steps = max(1, min(50, int(target_s / max(t1, 1e-3))))
+    t0 = time.perf_counter()
+    clib.simulate(9, S, C, 1, [tau_s, tau_a], steps)
+    dt = time.perf_counter() - t0
+    mlups = n * n * steps / dt / 1e6
+    return {"value": round(mlups, 2), "unit": "MLUPS", "cores": threads, "kind": "port",
+            "sample": f"D2Q9 TRT Taylor-Green {n}x{n}, FP64, {steps} steps "
+                      f"({dt:.1f} s) with the C restatement on {threads} OpenMP threads"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--n", type=int, default=4096)
+    ap.add_argument("--tau", type=float, default=0.6)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp16", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
+              "(the lattice shards as y-slabs in design only)", file=sys.stderr)
+        sys.exit(2)
+
+    import lbmhip as L
+    if L.device_count() < 1:
+        print("bench.py: no GPU visible", file=sys.stderr)
+        sys.exit(1)
+    n, K, W = a.n, a.steps, a.warmup
+    q = L.D2Q9()
+    cm = L.trt_magic(a.tau, 0.25)
+    print(f"[bench] building Taylor-Green {n}x{n} initial populations", file=sys.stderr,
+          flush=True)
+    f0 = tgv_populations(L, q, n)
+    nodes = n * n
+
+    print(f"[bench] FP64: {W} warmup + {K} timed steps", file=sys.stderr, flush=True)
+    m64, wall64, kms64, nl64 = run_storage(L, q, f0, cm, "f64", K, W)
+    mlups64 = nodes * K / wall64 / 1e6
+    bytes64 = 2 * q.Q * 8 * nodes
+    kavg64 = kms64 / max(nl64, 1)
+    gbs64 = bytes64 / (kavg64 * 1e-3) / 1e9
+    u64 = L.moments(m64.f_stream, stress=False)
+
+    result = {
+        "metric": METRIC,
+        "value": round(mlups64, 1),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": round(wall64 / K * 1e3, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"D2Q9 TRT (magic Lambda=1/4, tau={a.tau}) Taylor-Green vortex, "
+                        f"periodic {n}x{n}, plain FP64 populations, fused pull collide-stream",
+            "lattice": [n, n],
+            "quadrature": "D2Q9",
+            "collision": "TRT",
+            "tau_symmetric": cm.τ_symmetric,
+            "tau_asymmetric": cm.τ_asymmetric,
+            "parallelism": "single GPU",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(gbs64, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(gbs64 / HBM_PEAK_GBS, 4),
+            "traffic": traffic_for("d2q9_trt_f64_4096"),
+            "kernel": "k_fused<9,TRT,F64>",
+            "kernel_ms_avg": round(kavg64, 5),
+            "bytes_per_launch": bytes64,
+        },
+    }
+
+    if not a.no_fp16:
+        print(f"[bench] FP16-deviation: {W} warmup + {K} timed steps", file=sys.stderr,
+              flush=True)
+        m16, wall16, kms16, nl16 = run_storage(L, q, f0, cm, "f16dev", K, W)
+        mlups16 = nodes * K / wall16 / 1e6
+        bytes16 = 2 * q.Q * 2 * nodes
+        kavg16 = kms16 / max(nl16, 1)
+        gbs16 = bytes16 / (kavg16 * 1e-3) / 1e9
+        u16 = L.moments(m16.f_stream, stress=False)
+        num = np.sqrt(np.sum((u16.ux - u64.ux) ** 2 + (u16.uy - u64.uy) ** 2))
+        den = np.sqrt(np.sum(u64.ux ** 2 + u64.uy ** 2))
+        result["fp16dev"] = {
+            "value": round(mlups16, 1),
+            "unit": "MLUPS",
+            "ratio_vs_f64": round(mlups16 / mlups64, 3),
+            "rel_l2_velocity_err_vs_f64": float(num / den),
+            "steps_compared": W + K,
+            "ms_per_step": round(wall16 / K * 1e3, 5),
+            "roofline": {"bound": "hbm", "achieved": round(gbs16, 1), "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": round(gbs16 / HBM_PEAK_GBS, 4),
+                         "traffic": traffic_for("d2q9_trt_f16dev_4096"),
+                         "kernel": "k_fused<9,TRT,F16DEV>", "kernel_ms_avg": round(kavg16, 5),
+                         "bytes_per_launch": bytes16},
+        }
+        del m16
+
+    del m64
+    if not a.no_cpu:
+        print("[bench] CPU baseline (oracle restatement)", file=sys.stderr, flush=True)
+        result["cpu_baseline"] = cpu_baseline(n, cm.τ_symmetric, cm.τ_asymmetric, f0,
+                                              a.cpu_seconds)
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,140 @@
+/*
+ * lbm_hip.h -- C ABI of liblbm_hip.so, the MI355X (gfx950) collide-stream
+ * step of LatticeBoltzmann.jl.
+ *
+ * Every entry point is plain C: opaque handles, pointers and sizes, an int
+ * status (0 = ok, < 0 = error; lbm_last_error() gives the message).  A Julia
+ * binding reaches them with ccall; Python reaches them with ctypes.  Reference
+ * citations name the Julia function each entry point replaces (paths relative
+ * to the LatticeBoltzmann.jl repository).
+ *
+ * Host arrays are the reference's layout: a Julia Array{Float64,3} of size
+ * (nx, ny, Q), column-major, x fastest (src/initial_conditions.jl:13).  Device
+ * populations keep that layout (one contiguous HBM plane per velocity), so an
+ * FP64 upload or download is one memcpy.
+ *
+ * Ordering: all device work of one process is issued on one HIP stream owned
+ * by the library; uploads/downloads synchronise that stream.  Handles are not
+ * thread-safe; use one set of handles per host thread.
+ */
+#ifndef LBM_HIP_H
+#define LBM_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- enums ------------------------------------------------------------ */
+/* quadrature ids: src/quadratures.jl:30-38 (D2Q21 stores 25 populations) */
+enum { LBM_D2Q4 = 4, LBM_D2Q5 = 5, LBM_D2Q9 = 9, LBM_D2Q13 = 13,
+       LBM_D2Q17 = 17, LBM_D2Q21 = 21, LBM_D2Q37 = 37 };
+
+/* population storage in HBM.  F64 is the plain (reference-exact) form; the
+ * others are the reduced-precision forms with FP32 collision arithmetic. */
+enum { LBM_STORE_F64 = 0, LBM_STORE_F32 = 1, LBM_STORE_F16DEV = 2, LBM_STORE_BF16DEV = 3 };
+
+/* collision models: src/collision_models/{srt,trt,mrt}.jl */
+enum { LBM_SRT = 0, LBM_TRT = 1, LBM_MRT = 2 };
+
+/* body force, evaluated on the host (the reference's per-node closure
+ * srt.jl:71, trt.jl:158, mrt.jl:92 cannot cross the ABI) */
+enum { LBM_FORCE_NONE = 0, LBM_FORCE_CONST = 1, LBM_FORCE_FIELD = 2 };
+
+/* boundary conditions: src/boundary_conditions.jl:102-106 */
+enum { LBM_BOUNCE_BACK = 0, LBM_MOVING_WALL = 1 };
+enum { LBM_NORTH = 0, LBM_EAST = 1, LBM_SOUTH = 2, LBM_WEST = 3 };
+
+typedef struct {
+    int32_t kind;       /* LBM_BOUNCE_BACK or LBM_MOVING_WALL */
+    int32_t direction;  /* LBM_NORTH .. LBM_WEST (moving wall: North only) */
+    int32_t x0, x1;     /* bc.xs as a 1-based inclusive range */
+    int32_t y0, y1;     /* bc.ys as a 1-based inclusive range */
+    double ux, uy;      /* MovingWall u (moving_wall.jl:186-196) */
+    double rho;         /* MovingWall rho */
+} lbm_bc;
+
+typedef struct lbm_field_s *lbm_field;   /* one population array f (nx, ny, Q) on the GPU */
+typedef struct lbm_model_s *lbm_model;   /* quadrature + collision + force + BC list */
+
+/* ---- library ---------------------------------------------------------- */
+const char *lbm_last_error(void);
+int lbm_version(void);
+/* Number of visible GPUs; 0 without a GPU (does not create a context). */
+int lbm_device_count(void);
+/* Blocks until all work issued by the library has finished. */
+int lbm_synchronize(void);
+
+/* Quadrature constants as the library holds them (c as doubles, 2*Q).
+ * Replaces field access q.abscissae / q.weights / q.speed_of_sound_squared /
+ * order(q) / opposite(q, i) (src/quadratures.jl:11-19, quadratures/D2Q*.jl). */
+int lbm_quadrature_info(int quadrature, int *Q, double *cx, double *cy, double *w,
+                        double *s, int *order, int *opp /* 0-based */);
+
+/* ---- population fields (device arrays) -------------------------------- */
+/* Array{Float64}(undef, nx, ny, Q) on the device (initial_conditions.jl:13).
+ * rho0 is the rest density about which the *DEV storages store deviations. */
+int lbm_field_create(lbm_field *out, int quadrature, int nx, int ny, int storage, double rho0);
+int lbm_field_destroy(lbm_field f);
+int lbm_field_info(lbm_field f, int *quadrature, int *nx, int *ny, int *Q, int *storage);
+/* copy a host (nx, ny, Q) column-major Float64 array in / out (converting for
+ * reduced storages).  Synchronous. */
+int lbm_field_upload(lbm_field f, const double *host);
+int lbm_field_download(lbm_field f, double *host);
+/* dst .= src (same shape and storage), a device copy kernel */
+int lbm_field_copy(lbm_field dst, lbm_field src);
+
+/* ---- model (collision model, force, boundary conditions) --------------- */
+int lbm_model_create(lbm_model *out, int quadrature);
+int lbm_model_destroy(lbm_model m);
+/* SRT(tau) -- srt.jl:20-24 */
+int lbm_model_set_srt(lbm_model m, double tau);
+/* TRT(tau_s, tau_a, force) field order -- trt.jl:82-87 (note the reference's
+ * 2-argument TRT(a, b) means tau_a = a, tau_s = b; the binding must swap) */
+int lbm_model_set_trt(lbm_model m, double tau_symmetric, double tau_asymmetric);
+/* MRT(q, taus) -- mrt.jl:19-34; taus[n-1] relaxes Hermite order n */
+int lbm_model_set_mrt(lbm_model m, const double *taus, int ntaus);
+/* LBM_FORCE_NONE; LBM_FORCE_CONST: values = {Fx, Fy}; LBM_FORCE_FIELD:
+ * values = (nx, ny, 2) column-major lattice force, copied to the device. */
+int lbm_model_set_force(lbm_model m, int kind, const double *values, int nx, int ny);
+/* ordered list, applied in order (boundary_conditions.jl:80-90) */
+int lbm_model_set_bcs(lbm_model m, const lbm_bc *bcs, int n);
+
+/* ---- step operators ----------------------------------------------------- */
+/* collide!(cm, q, f_in, f_out; time) -- collision_models.jl:19, srt.jl:37-81,
+ * trt.jl:123-178, mrt.jl:49-118.  f_in and f_out must be distinct. */
+int lbm_collide(lbm_model m, lbm_field f_in, lbm_field f_out);
+/* stream!(q, f, f_new) -- stream.jl:18-30 (periodic pull, mod1 wrap) */
+int lbm_stream(lbm_model m, lbm_field f, lbm_field f_new);
+/* apply!(bcs, q, f_new, f_old; time) -- boundary_conditions.jl:80-94,
+ * bounce_back.jl:117-181, moving_wall.jl:198-219.  Overwrites f_new in place. */
+int lbm_apply_bcs(lbm_model m, lbm_field f_new, lbm_field f_old);
+/* stream! then apply!, as one pass */
+int lbm_stream_bc(lbm_model m, lbm_field f, lbm_field f_new);
+/* simulate(model, t0:t0+nsteps-1) without processing -- lattice_boltzmann_model.jl:60-77:
+ * nsteps x (collide! f_stream -> f_collision; stream! + apply!) with one fused
+ * pull collide-stream kernel per step.  On return both buffers hold exactly
+ * what the reference's model.f_stream / model.f_collision hold. */
+int lbm_simulate(lbm_model m, lbm_field f_stream, lbm_field f_collision, int64_t nsteps);
+/* As lbm_simulate, also returning the time of the fused kernels (HIP events on
+ * the library stream): *ms_total over all nsteps, *nlaunch fused launches. */
+int lbm_simulate_timed(lbm_model m, lbm_field f_stream, lbm_field f_collision, int64_t nsteps,
+                       double *ms_total, int64_t *nlaunch);
+
+/* ---- moments ------------------------------------------------------------ */
+/* density / velocity! / deviatoric_tensor per node (moments.jl:1-19,199-214),
+ * as TrackHydrodynamicErrors uses them (track_hydrodynamic_errors.jl:139-177).
+ * Outputs are (nx, ny) column-major Float64 host arrays; any may be NULL. */
+int lbm_moments(lbm_field f, double tau, double *rho, double *ux, double *uy,
+                double *sxx, double *sxy, double *syy);
+/* FP64 global sums over the lattice (track_hydrodynamic_errors.jl:192-202):
+ * out[0] = sum rho, out[1] = sum rho ux, out[2] = sum rho uy,
+ * out[3] = sum rho |u|^2, out[4] = number of nodes with a non-finite population */
+int lbm_reduce(lbm_field f, double *out5);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,289 @@
+// kernels_aux.hip -- moments, FP64 reductions and storage conversion.
+// Compiled with -ffp-contract=off (FP64 arithmetic in the reference's order).
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+#include <hip/hip_bf16.h>
+
+#include "lbm_internal.h"
+#include "lbm_quad.h"
+
+namespace lbm {
+
+// population i at linear node n, reconstructed in FP64 from any storage
+template <int SID>
+__device__ __forceinline__ double load_d(const void *base, long long plane, long long n, int i,
+                                         const StepParams<double> &p) {
+    const long long k = plane * i + n;
+    if constexpr (SID == 0) {
+        return static_cast<const double *>(base)[k];
+    } else if constexpr (SID == 1) {
+        return static_cast<double>(static_cast<const float *>(base)[k]);
+    } else if constexpr (SID == 2) {
+        const double g = static_cast<double>(__half2float(static_cast<const __half *>(base)[k]));
+        return p.w[i] * p.rho0 + g * p.dscale_inv;
+    } else {
+        const double g =
+            static_cast<double>(__bfloat162float(static_cast<const __hip_bfloat16 *>(base)[k]));
+        return p.w[i] * p.rho0 + g * p.dscale_inv;
+    }
+}
+
+template <int SID>
+__device__ __forceinline__ void store_d(void *base, long long plane, long long n, int i, double v,
+                                        const StepParams<double> &p) {
+    const long long k = plane * i + n;
+    if constexpr (SID == 0) {
+        static_cast<double *>(base)[k] = v;
+    } else if constexpr (SID == 1) {
+        static_cast<float *>(base)[k] = static_cast<float>(v);
+    } else if constexpr (SID == 2) {
+        const double g = (v - p.w[i] * p.rho0) * p.dscale;
+        static_cast<__half *>(base)[k] = __float2half_rn(static_cast<float>(g));
+    } else {
+        const double g = (v - p.w[i] * p.rho0) * p.dscale;
+        static_cast<__hip_bfloat16 *>(base)[k] = __float2bfloat16(static_cast<float>(g));
+    }
+}
+
+// host Float64 (nx, ny, Q) <-> device storage, elementwise
+template <int SID>
+__global__ void __launch_bounds__(256)
+k_convert(const double *__restrict__ hostlike, void *dev, int Q, int to_device,
+          double *__restrict__ out, const StepParams<double> p) {
+    const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= p.plane) return;
+    for (int i = 0; i < Q; ++i) {
+        if (to_device) store_d<SID>(dev, p.plane, n, i, hostlike[p.plane * i + n], p);
+        else out[p.plane * i + n] = load_d<SID>(dev, p.plane, n, i, p);
+    }
+}
+
+// density / velocity! / deviatoric_tensor per node, in the CPU restatement's order
+// (moments.jl:1-19, 199-214; hermite_polynomials.jl:124-128; hermite.jl:45-48)
+template <int QID, int SID>
+__global__ void __launch_bounds__(256)
+k_moments(const void *f, const StepParams<double> p, double tau, double *__restrict__ rho_o,
+          double *__restrict__ ux_o, double *__restrict__ uy_o, double *__restrict__ sxx_o,
+          double *__restrict__ sxy_o, double *__restrict__ syy_o) {
+    constexpr int Q = QT<QID>::Q;
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y;
+    if (x >= p.nx) return;
+    const long long n = (long long)x + (long long)p.nx * y;
+    double fl[Q];
+#pragma unroll
+    for (int i = 0; i < Q; ++i) fl[i] = load_d<SID>(f, p.plane, n, i, p);
+    double rho = fl[0] + fl[1];
+#pragma unroll
+    for (int i = 2; i < Q; ++i) rho = rho + fl[i];
+    double ux = 0.0, uy = 0.0;
+#pragma unroll
+    for (int i = 0; i < Q; ++i) ux = ux + fl[i] * static_cast<double>(QT<QID>::cx(i));
+    ux = ux / rho;
+#pragma unroll
+    for (int i = 0; i < Q; ++i) uy = uy + fl[i] * static_cast<double>(QT<QID>::cy(i));
+    uy = uy / rho;
+    if (rho_o) rho_o[n] = rho;
+    if (ux_o) ux_o[n] = ux;
+    if (uy_o) uy_o[n] = uy;
+    if (!(sxx_o || sxy_o || syy_o)) return;
+    const double cs = p.cs_inv;
+    auto h2 = [&](int i, int k) -> double {
+        const int a = k & 1, b = k >> 1;
+        const int ca = a == 0 ? QT<QID>::cx(i) : QT<QID>::cy(i);
+        const int cb = b == 0 ? QT<QID>::cx(i) : QT<QID>::cy(i);
+        return static_cast<double>(cb * ca) - cs * static_cast<double>(a == b ? 1 : 0);
+    };
+    double abar[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        double a = fl[0] * h2(0, k) + fl[1] * h2(1, k);
+#pragma unroll
+        for (int i = 2; i < Q; ++i) a = a + fl[i] * h2(i, k);
+        abar[k] = a;
+    }
+    const double u[2] = {ux, uy};
+    double sg[4];
+    const double den = 1.0 + 1.0 / (2.0 * tau);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const double aeq = rho * (u[k & 1] * u[k >> 1] + 0.0);
+        sg[k] = (abar[k] - aeq) / den;
+    }
+    const double tr2 = (sg[0] + sg[3]) / 2.0;
+    if (sxx_o) sxx_o[n] = sg[0] - tr2;
+    if (sxy_o) sxy_o[n] = sg[2];
+    if (syy_o) syy_o[n] = sg[3] - tr2;
+}
+
+// per-block FP64 partial sums: rho, rho ux, rho uy, rho |u|^2, #non-finite nodes
+template <int QID, int SID>
+__global__ void __launch_bounds__(256)
+k_reduce_partial(const void *f, const StepParams<double> p, double *__restrict__ partials) {
+    constexpr int Q = QT<QID>::Q;
+    __shared__ double red[5][256];
+    const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    if (n < p.plane) {
+        double fl[Q];
+        bool finite = true;
+#pragma unroll
+        for (int i = 0; i < Q; ++i) {
+            fl[i] = load_d<SID>(f, p.plane, n, i, p);
+            finite = finite && isfinite(fl[i]);
+        }
+        double rho = fl[0] + fl[1];
+#pragma unroll
+        for (int i = 2; i < Q; ++i) rho = rho + fl[i];
+        double ux = 0.0, uy = 0.0;
+#pragma unroll
+        for (int i = 0; i < Q; ++i) ux = ux + fl[i] * static_cast<double>(QT<QID>::cx(i));
+        ux = ux / rho;
+#pragma unroll
+        for (int i = 0; i < Q; ++i) uy = uy + fl[i] * static_cast<double>(QT<QID>::cy(i));
+        uy = uy / rho;
+        if (finite) {
+            v[0] = rho;
+            v[1] = rho * ux;
+            v[2] = rho * uy;
+            v[3] = rho * (ux * ux + uy * uy);
+        } else {
+            v[4] = 1.0;
+        }
+    }
+    for (int k = 0; k < 5; ++k) red[k][threadIdx.x] = v[k];
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int k = 0; k < 5; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 5; ++k) partials[(long long)blockIdx.x * 5 + k] = red[k][0];
+}
+
+// fixed-order final sum of the partials (one block): deterministic
+__global__ void __launch_bounds__(256)
+k_reduce_final(const double *__restrict__ partials, int npart, double *__restrict__ out) {
+    __shared__ double red[5][256];
+    double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int b = threadIdx.x; b < npart; b += blockDim.x)
+        for (int k = 0; k < 5; ++k) v[k] += partials[(long long)b * 5 + k];
+    for (int k = 0; k < 5; ++k) red[k][threadIdx.x] = v[k];
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int k = 0; k < 5; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 5; ++k) out[k] = red[k][0];
+}
+
+template <class U>
+__global__ void __launch_bounds__(256) k_copy_aux(const U *__restrict__ src, U *__restrict__ dst,
+                                                  long long n) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long k = i; k < n; k += stride) dst[k] = src[k];
+}
+
+// ---------------------------------------------------------------------------
+template <int QID>
+static int moments_q(int storage, const void *f, const StepParams<double> &p, double tau,
+                     double *o0, double *o1, double *o2, double *o3, double *o4, double *o5,
+                     hipStream_t s) {
+    const int bx = p.nx >= 256 ? 256 : 64;
+    dim3 grid((p.nx + bx - 1) / bx, p.ny, 1), block(bx, 1, 1);
+    switch (storage) {
+    case 0: k_moments<QID, 0><<<grid, block, 0, s>>>(f, p, tau, o0, o1, o2, o3, o4, o5); break;
+    case 1: k_moments<QID, 1><<<grid, block, 0, s>>>(f, p, tau, o0, o1, o2, o3, o4, o5); break;
+    case 2: k_moments<QID, 2><<<grid, block, 0, s>>>(f, p, tau, o0, o1, o2, o3, o4, o5); break;
+    case 3: k_moments<QID, 3><<<grid, block, 0, s>>>(f, p, tau, o0, o1, o2, o3, o4, o5); break;
+    default: return -1;
+    }
+    return hipGetLastError() == hipSuccess ? 0 : -4;
+}
+
+int dispatch_moments(int qid, int storage, const void *f, const StepParams<double> &p, double tau,
+                     double *rho, double *ux, double *uy, double *sxx, double *sxy, double *syy,
+                     hipStream_t s) {
+    switch (qid) {
+    case 4: return moments_q<4>(storage, f, p, tau, rho, ux, uy, sxx, sxy, syy, s);
+    case 5: return moments_q<5>(storage, f, p, tau, rho, ux, uy, sxx, sxy, syy, s);
+    case 9: return moments_q<9>(storage, f, p, tau, rho, ux, uy, sxx, sxy, syy, s);
+    case 13: return moments_q<13>(storage, f, p, tau, rho, ux, uy, sxx, sxy, syy, s);
+    case 17: return moments_q<17>(storage, f, p, tau, rho, ux, uy, sxx, sxy, syy, s);
+    case 21: return moments_q<21>(storage, f, p, tau, rho, ux, uy, sxx, sxy, syy, s);
+    case 37: return moments_q<37>(storage, f, p, tau, rho, ux, uy, sxx, sxy, syy, s);
+    default: return -1;
+    }
+}
+
+template <int QID>
+static int reduce_q(int storage, const void *f, const StepParams<double> &p, double *partials,
+                    int npart, hipStream_t s) {
+    dim3 grid(npart, 1, 1), block(256, 1, 1);
+    switch (storage) {
+    case 0: k_reduce_partial<QID, 0><<<grid, block, 0, s>>>(f, p, partials); break;
+    case 1: k_reduce_partial<QID, 1><<<grid, block, 0, s>>>(f, p, partials); break;
+    case 2: k_reduce_partial<QID, 2><<<grid, block, 0, s>>>(f, p, partials); break;
+    case 3: k_reduce_partial<QID, 3><<<grid, block, 0, s>>>(f, p, partials); break;
+    default: return -1;
+    }
+    return hipGetLastError() == hipSuccess ? 0 : -4;
+}
+
+int dispatch_reduce(int qid, int storage, const void *f, const StepParams<double> &p,
+                    double *partials, int npart, double *out5, hipStream_t s) {
+    int rc;
+    switch (qid) {
+    case 4: rc = reduce_q<4>(storage, f, p, partials, npart, s); break;
+    case 5: rc = reduce_q<5>(storage, f, p, partials, npart, s); break;
+    case 9: rc = reduce_q<9>(storage, f, p, partials, npart, s); break;
+    case 13: rc = reduce_q<13>(storage, f, p, partials, npart, s); break;
+    case 17: rc = reduce_q<17>(storage, f, p, partials, npart, s); break;
+    case 21: rc = reduce_q<21>(storage, f, p, partials, npart, s); break;
+    case 37: rc = reduce_q<37>(storage, f, p, partials, npart, s); break;
+    default: return -1;
+    }
+    if (rc) return rc;
+    k_reduce_final<<<1, 256, 0, s>>>(partials, npart, out5);
+    return hipGetLastError() == hipSuccess ? 0 : -4;
+}
+
+int dispatch_upload(int qid, int storage, const double *src, void *dst,
+                    const StepParams<double> &p, int to_device, hipStream_t s) {
+    int Q = qid == 21 ? 25 : qid;
+    const long long nodes = p.plane;
+    const int blocks = (int)((nodes + 255) / 256);
+    double *out = const_cast<double *>(src);
+    switch (storage) {
+    case 0: k_convert<0><<<blocks, 256, 0, s>>>(src, dst, Q, to_device, out, p); break;
+    case 1: k_convert<1><<<blocks, 256, 0, s>>>(src, dst, Q, to_device, out, p); break;
+    case 2: k_convert<2><<<blocks, 256, 0, s>>>(src, dst, Q, to_device, out, p); break;
+    case 3: k_convert<3><<<blocks, 256, 0, s>>>(src, dst, Q, to_device, out, p); break;
+    default: return -1;
+    }
+    return hipGetLastError() == hipSuccess ? 0 : -4;
+}
+
+// device copy of nbytes (multiple of 8 for double storage, 4 float, 2 half)
+int dispatch_copy(const void *src, void *dst, size_t nbytes, hipStream_t s) {
+    if (nbytes % 8 == 0) {
+        const long long n = (long long)(nbytes / 8);
+        long long blocks = (n + 255) / 256;
+        if (blocks > 65536) blocks = 65536;
+        k_copy_aux<double><<<(int)blocks, 256, 0, s>>>(static_cast<const double *>(src),
+                                                        static_cast<double *>(dst), n);
+    } else {
+        const long long n = (long long)(nbytes / 2);
+        long long blocks = (n + 255) / 256;
+        if (blocks > 65536) blocks = 65536;
+        k_copy_aux<unsigned short><<<(int)blocks, 256, 0, s>>>(
+            static_cast<const unsigned short *>(src), static_cast<unsigned short *>(dst), n);
+    }
+    return hipGetLastError() == hipSuccess ? 0 : -4;
+}
+
+}  // namespace lbm

@@ -1,0 +1,757 @@
+// lbm_capi.cpp -- C ABI of liblbm_hip.so (declared in include/lbm_hip.h).
+//
+// Host-side logic: handles, validation, parameter packing and the step
+// schedule of lbm_simulate.  All device work goes to one HIP stream owned by
+// the library.  Compiled with -ffp-contract=off so that the constants it
+// precomputes (1/tau, s*s, moving-wall terms, ...) round exactly as the
+// reference's per-node expressions do.
+#include "lbm_hip.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "lbm_internal.h"
+#include "lbm_quad.h"
+
+namespace {
+
+thread_local std::string g_err;
+hipStream_t g_stream = nullptr;
+
+int fail(int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                     \
+    do {                                                                                  \
+        hipError_t e_ = (expr);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            return fail(-3, "%s failed: %s", #expr, hipGetErrorString(e_));               \
+    } while (0)
+
+int ensure_stream() {
+    if (g_stream) return 0;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n < 1)
+        return fail(-3, "no HIP device visible (liblbm_hip needs an MI355X / gfx950 GPU)");
+    HIP_TRY(hipStreamCreate(&g_stream));
+    return 0;
+}
+
+// x^4 rounded once from an exact two-product (the same sequence as the kernels)
+double pow4(double x) {
+    double p = x * x;
+    double e = std::fma(x, x, -p);
+    double hi = p * p;
+    double lo = std::fma(p, p, -hi);
+    return hi + (lo + (2.0 * p) * e);
+}
+
+struct HostQuad {
+    int id = 0, Q = 0, order = 0, maxc = 0;
+    int cx[37], cy[37], opp[37];
+    double w[37];
+    double s = 0.0;
+};
+
+template <int QID>
+void fill_tables(HostQuad &q) {
+    using QTq = lbm::QT<QID>;
+    q.Q = QTq::Q;
+    q.order = QTq::ORDER;
+    q.maxc = QTq::MAXC;
+    for (int i = 0; i < QTq::Q; ++i) {
+        q.cx[i] = QTq::cx(i);
+        q.cy[i] = QTq::cy(i);
+        q.opp[i] = QTq::opp(i);
+    }
+}
+
+// Weights and speed_of_sound_squared with the reference's expressions
+// (src/quadratures/D2Q{4,5,9,13,17,21,37}.jl).
+bool host_quad(int qid, HostQuad &q) {
+    q = HostQuad{};
+    q.id = qid;
+    switch (qid) {
+    case 4:
+        fill_tables<4>(q);
+        for (int i = 0; i < 4; ++i) q.w[i] = 1.0 / 4.0;
+        q.s = 2.0;
+        return true;
+    case 5:
+        fill_tables<5>(q);
+        q.w[0] = 4.0 / 6.0;
+        for (int i = 1; i < 5; ++i) q.w[i] = 1.0 / 12.0;
+        q.s = 6.0;
+        return true;
+    case 9: {
+        fill_tables<9>(q);
+        const double w[9] = {4.0 / 9, 1.0 / 36, 1.0 / 9, 1.0 / 36, 1.0 / 9,
+                             1.0 / 36, 1.0 / 9, 1.0 / 36, 1.0 / 9};
+        for (int i = 0; i < 9; ++i) q.w[i] = w[i];
+        q.s = 3.0;
+        return true;
+    }
+    case 13: {
+        fill_tables<13>(q);
+        const double a = 3.0 / 8, b = 1.0 / 12, c = 1.0 / 16, d = 1.0 / 96;
+        const double w[13] = {a, b, b, b, b, c, c, c, c, d, d, d, d};
+        for (int i = 0; i < 13; ++i) q.w[i] = w[i];
+        q.s = 2.0;
+        return true;
+    }
+    case 17: {
+        fill_tables<17>(q);
+        const double sq = std::sqrt(193.0);
+        const double g[5] = {(575.0 + 193.0 * sq) / 8100.0, (3355.0 - 91.0 * sq) / 18000.0,
+                             (655.0 + 17.0 * sq) / 27000.0, (685.0 - 49.0 * sq) / 54000.0,
+                             (1445.0 - 101.0 * sq) / 162000.0};
+        q.w[0] = g[0];
+        for (int i = 1; i < 17; ++i) q.w[i] = g[1 + (i - 1) / 4];
+        q.s = (125.0 + 5.0 * std::sqrt(193.0)) / 72.0;
+        return true;
+    }
+    case 21: {
+        fill_tables<21>(q);
+        const double g[7] = {91.0 / 324, 1.0 / 12, 2.0 / 27, 7.0 / 360, 1.0 / 432, 1.0 / 1620, 0.0};
+        q.w[0] = g[0];
+        for (int i = 1; i < 25; ++i) q.w[i] = g[1 + (i - 1) / 4];
+        q.s = 3.0 / 2.0;
+        return true;
+    }
+    case 37: {
+        fill_tables<37>(q);
+        const double g[8] = {0.23315066913235250228650, 0.10730609154221900241246,
+                             0.05766785988879488203006, 0.01420821615845075026469,
+                             0.00535304900051377523273, 0.00101193759267357547541,
+                             0.00024530102775771734547, 0.00028341425299419821740};
+        const int grp[37] = {0, 1, 1, 1, 1, 2, 2, 2, 2, 3, 3, 3, 3, 4, 4, 4, 4, 4, 4,
+                             4, 4, 5, 5, 5, 5, 6, 6, 6, 6, 7, 7, 7, 7, 7, 7, 7, 7};
+        for (int i = 0; i < 37; ++i) q.w[i] = g[grp[i]];
+        const double r = 1.19697977039307435897239;
+        q.s = r * r;
+        return true;
+    }
+    default:
+        return false;
+    }
+}
+
+int storage_bytes(int storage) {
+    switch (storage) {
+    case LBM_STORE_F64: return 8;
+    case LBM_STORE_F32: return 4;
+    case LBM_STORE_F16DEV:
+    case LBM_STORE_BF16DEV: return 2;
+    default: return 0;
+    }
+}
+
+// Hermite tensor H_n(c) scaled by cs = 1/s (src/hermite_polynomials.jl:124-159),
+// flattened column-major.
+int delta(int a, int b) { return a == b ? 1 : 0; }
+void hermite_tensor(int n, int cx, int cy, double cs, double *H) {
+    const int xi[2] = {cx, cy};
+    const int m = 1 << n;
+    for (int k = 0; k < m; ++k) {
+        int ix[4] = {k & 1, (k >> 1) & 1, (k >> 2) & 1, (k >> 3) & 1};
+        if (n == 2) {
+            int a = ix[0], b = ix[1];
+            H[k] = (double)(xi[b] * xi[a]) - cs * (double)delta(a, b);
+        } else if (n == 3) {
+            int a = ix[0], b = ix[1], g = ix[2];
+            int A = xi[g] * xi[b] * xi[a];
+            int B = xi[a] * delta(b, g) + xi[b] * delta(a, g) + xi[g] * delta(a, b);
+            H[k] = (double)A - cs * (double)B;
+        } else {
+            int a = ix[0], b = ix[1], g = ix[2], e = ix[3];
+            int A = xi[e] * xi[g] * xi[b] * xi[a];
+            int B = xi[a] * xi[b] * delta(g, e) + xi[a] * xi[g] * delta(b, e) +
+                    xi[a] * xi[e] * delta(b, g) + xi[b] * xi[g] * delta(a, e) +
+                    xi[b] * xi[e] * delta(a, g) + xi[g] * xi[e] * delta(a, b);
+            int C = delta(a, b) * delta(g, e) + delta(a, g) * delta(b, e) +
+                    delta(a, e) * delta(b, g);
+            H[k] = ((double)A - cs * (double)B) + (cs * cs) * (double)C;
+        }
+    }
+}
+
+}  // namespace
+
+struct lbm_field_s {
+    int qid = 0, nx = 0, ny = 0, Q = 0, storage = 0;
+    double rho0 = 1.0, dscale = 1.0;
+    void *d = nullptr;
+    size_t bytes = 0;
+};
+
+struct lbm_model_s {
+    int qid = 0;
+    HostQuad q;
+    int coll = LBM_SRT;
+    double tau = 1.0, tau_s = 1.0, tau_a = 1.0;
+    std::vector<double> taus;
+    int force_kind = LBM_FORCE_NONE;
+    double F[2] = {0.0, 0.0};
+    double *dF = nullptr;
+    int fnx = 0, fny = 0;
+    std::vector<lbm_bc> bcs;
+    lbm::BCDev *dB = nullptr;
+    double *dMw = nullptr;
+    double *dH = nullptr;
+    double *dPart = nullptr;  // reduction partials
+    size_t npart = 0;
+    double *dOut = nullptr;
+};
+
+namespace {
+
+int check_field(lbm_field f, const char *what) {
+    if (!f || !f->d) return fail(-1, "%s: null field", what);
+    return 0;
+}
+
+int check_pair(lbm_model m, lbm_field a, lbm_field b, const char *what, bool uses_force) {
+    if (!m) return fail(-1, "%s: null model", what);
+    if (int rc = check_field(a, what)) return rc;
+    if (int rc = check_field(b, what)) return rc;
+    if (a == b || a->d == b->d)
+        return fail(-1, "%s: input and output populations must be distinct arrays", what);
+    if (a->qid != m->qid || b->qid != m->qid)
+        return fail(-1, "%s: field quadrature D2Q%d/D2Q%d does not match the model's D2Q%d", what,
+                    a->qid, b->qid, m->qid);
+    if (a->nx != b->nx || a->ny != b->ny || a->storage != b->storage)
+        return fail(-1, "%s: fields differ in shape or storage", what);
+    if (uses_force && m->force_kind == LBM_FORCE_FIELD && (m->fnx != a->nx || m->fny != a->ny))
+        return fail(-1, "%s: force field is %dx%d but the lattice is %dx%d", what, m->fnx,
+                    m->fny, a->nx, a->ny);
+    for (const lbm_bc &b : m->bcs) {
+        if (b.kind == LBM_MOVING_WALL) continue;  // acts on the whole lattice
+        const bool xe = b.x1 < b.x0, ye = b.y1 < b.y0;
+        if ((!xe && (b.x0 < 1 || b.x1 > a->nx)) || (!ye && (b.y0 < 1 || b.y1 > a->ny)))
+            return fail(-1, "%s: boundary range x %d:%d, y %d:%d outside the %dx%d lattice", what,
+                        b.x0, b.x1, b.y0, b.y1, a->nx, a->ny);
+    }
+    if (uses_force && m->coll == LBM_MRT && a->storage != LBM_STORE_F64)
+        return fail(-2, "%s: MRT is implemented for FP64 storage only", what);
+    return 0;
+}
+
+lbm::StepParams<double> make_params(const lbm_model_s *m, const lbm_field_s *f, bool with_bcs) {
+    lbm::StepParams<double> p{};
+    const HostQuad &q = m->q;
+    p.nx = f->nx;
+    p.ny = f->ny;
+    p.plane = (long long)f->nx * (long long)f->ny;
+    for (int i = 0; i < 37; ++i) p.w[i] = i < q.Q ? q.w[i] : 0.0;
+    const double s = q.s;
+    p.s = s;
+    p.neg_s = -s;
+    p.s2 = s * s;
+    p.three_s = 3.0 * s;
+    p.six_s3 = 6.0 * ((s * s) * s);
+    p.three_s2 = 3.0 * (s * s);
+    p.s4 = pow4(s);
+    p.cs_inv = 1.0 / s;
+    if (m->coll == LBM_SRT) {
+        p.omo = 1.0 - 1.0 / m->tau;
+        p.om = 1.0 / m->tau;
+        p.tau_force = m->tau;
+    } else if (m->coll == LBM_TRT) {
+        p.oms = 1.0 / m->tau_s;
+        p.oma = 1.0 / m->tau_a;
+        p.tau_force = m->tau_a;
+    } else {
+        p.mrt_n = q.order / 2;
+        for (int n = 1; n <= 4; ++n) {
+            const double t = (size_t)(n - 1) < m->taus.size() ? m->taus[n - 1] : 1.0;
+            p.mrt_omo[n] = 1.0 - 1.0 / t;
+            p.mrt_om[n] = 1.0 / t;
+            p.mrt_sn[n] = std::pow(s, (double)n);
+        }
+        p.tau_force = m->taus.size() > 1 ? m->taus[1] : 1.0;
+        p.mrt_H = m->dH;
+    }
+    p.force_kind = m->force_kind;
+    p.Fx = m->F[0];
+    p.Fy = m->F[1];
+    p.Ffield = m->dF;
+    p.nbc = with_bcs ? (int)m->bcs.size() : 0;
+    p.bcs = m->dB;
+    p.mw = m->dMw;
+    p.rho0 = f->rho0;
+    p.dscale = f->dscale;
+    p.dscale_inv = 1.0 / f->dscale;
+    return p;
+}
+
+int run(lbm::Op op, lbm_model m, lbm_field src, lbm_field dst, bool with_bcs) {
+    lbm::Launch L{};
+    L.op = op;
+    L.qid = m->qid;
+    L.coll = m->coll;
+    L.storage = src->storage;
+    L.src = src->d;
+    L.dst = dst->d;
+    const lbm::StepParams<double> p = make_params(m, src, with_bcs);
+    const int rc = src->storage == LBM_STORE_F64 ? lbm::dispatch_f64(L, p, g_stream)
+                                                 : lbm::dispatch_lp(L, p, g_stream);
+    if (rc) return fail(-4, "kernel launch failed (code %d): %s", rc,
+                        hipGetErrorString(hipGetLastError()));
+    return 0;
+}
+
+void free_dev(void *&p) {
+    if (p) {
+        (void)hipFree(p);
+        p = nullptr;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *lbm_last_error(void) { return g_err.c_str(); }
+
+int lbm_version(void) { return 10000; }
+
+int lbm_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int lbm_synchronize(void) {
+    if (!g_stream) return 0;
+    HIP_TRY(hipStreamSynchronize(g_stream));
+    return 0;
+}
+
+int lbm_quadrature_info(int quadrature, int *Q, double *cx, double *cy, double *w, double *s,
+                        int *order, int *opp) {
+    HostQuad q;
+    if (!host_quad(quadrature, q)) return fail(-1, "unknown quadrature D2Q%d", quadrature);
+    if (Q) *Q = q.Q;
+    for (int i = 0; i < q.Q; ++i) {
+        if (cx) cx[i] = (double)q.cx[i];
+        if (cy) cy[i] = (double)q.cy[i];
+        if (w) w[i] = q.w[i];
+        if (opp) opp[i] = q.opp[i];
+    }
+    if (s) *s = q.s;
+    if (order) *order = q.order;
+    return 0;
+}
+
+// ---- fields -----------------------------------------------------------------
+int lbm_field_create(lbm_field *out, int quadrature, int nx, int ny, int storage, double rho0) {
+    if (!out) return fail(-1, "lbm_field_create: null output handle");
+    *out = nullptr;
+    HostQuad q;
+    if (!host_quad(quadrature, q)) return fail(-1, "unknown quadrature D2Q%d", quadrature);
+    if (nx < 1 || ny < 1) return fail(-1, "lattice size %dx%d must be positive", nx, ny);
+    const int eb = storage_bytes(storage);
+    if (!eb) return fail(-1, "unknown storage %d", storage);
+    if ((long long)nx * (long long)ny > (1LL << 31) - 1)
+        return fail(-1, "lattice of %lld nodes exceeds 2^31-1", (long long)nx * ny);
+    if (int rc = ensure_stream()) return rc;
+    lbm_field f = new lbm_field_s;
+    f->qid = quadrature;
+    f->nx = nx;
+    f->ny = ny;
+    f->Q = q.Q;
+    f->storage = storage;
+    f->rho0 = rho0;
+    // deviation storages scale g = f - w rho0 by a power of two (exact)
+    f->dscale = (storage == LBM_STORE_F16DEV || storage == LBM_STORE_BF16DEV) ? 1.0 : 1.0;
+    f->bytes = (size_t)nx * (size_t)ny * (size_t)q.Q * (size_t)eb;
+    hipError_t e = hipMalloc(&f->d, f->bytes);
+    if (e != hipSuccess) {
+        delete f;
+        return fail(-3, "hipMalloc(%zu) failed: %s", (size_t)nx * ny * q.Q * eb,
+                    hipGetErrorString(e));
+    }
+    e = hipMemsetAsync(f->d, 0, f->bytes, g_stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(g_stream);
+    if (e != hipSuccess) {
+        (void)hipFree(f->d);
+        delete f;
+        return fail(-3, "clearing a new field failed: %s", hipGetErrorString(e));
+    }
+    *out = f;
+    return 0;
+}
+
+int lbm_field_destroy(lbm_field f) {
+    if (!f) return 0;
+    if (g_stream) (void)hipStreamSynchronize(g_stream);
+    free_dev(f->d);
+    delete f;
+    return 0;
+}
+
+int lbm_field_info(lbm_field f, int *quadrature, int *nx, int *ny, int *Q, int *storage) {
+    if (!f) return fail(-1, "lbm_field_info: null field");
+    if (quadrature) *quadrature = f->qid;
+    if (nx) *nx = f->nx;
+    if (ny) *ny = f->ny;
+    if (Q) *Q = f->Q;
+    if (storage) *storage = f->storage;
+    return 0;
+}
+
+static int transfer(lbm_field f, double *host, bool up) {
+    if (int rc = check_field(f, up ? "lbm_field_upload" : "lbm_field_download")) return rc;
+    if (!host) return fail(-1, "null host array");
+    const size_t nbytes64 = (size_t)f->nx * f->ny * f->Q * sizeof(double);
+    if (f->storage == LBM_STORE_F64) {
+        HIP_TRY(hipMemcpyAsync(up ? f->d : (void *)host, up ? (const void *)host : f->d, nbytes64,
+                               up ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost, g_stream));
+        HIP_TRY(hipStreamSynchronize(g_stream));
+        return 0;
+    }
+    double *tmp = nullptr;
+    HIP_TRY(hipMalloc(&tmp, nbytes64));
+    lbm::StepParams<double> p{};
+    HostQuad q;
+    host_quad(f->qid, q);
+    p.nx = f->nx;
+    p.ny = f->ny;
+    p.plane = (long long)f->nx * f->ny;
+    for (int i = 0; i < 37; ++i) p.w[i] = i < q.Q ? q.w[i] : 0.0;
+    p.rho0 = f->rho0;
+    p.dscale = f->dscale;
+    p.dscale_inv = 1.0 / f->dscale;
+    hipError_t e = hipSuccess;
+    if (up) e = hipMemcpyAsync(tmp, host, nbytes64, hipMemcpyHostToDevice, g_stream);
+    int rc = 0;
+    if (e == hipSuccess) rc = lbm::dispatch_upload(f->qid, f->storage, tmp, f->d, p, up ? 1 : 0,
+                                                   g_stream);
+    if (e == hipSuccess && !rc && !up)
+        e = hipMemcpyAsync(host, tmp, nbytes64, hipMemcpyDeviceToHost, g_stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(g_stream);
+    (void)hipFree(tmp);
+    if (e != hipSuccess) return fail(-3, "transfer failed: %s", hipGetErrorString(e));
+    if (rc) return fail(-4, "conversion kernel failed (%d)", rc);
+    return 0;
+}
+
+int lbm_field_upload(lbm_field f, const double *host) {
+    return transfer(f, const_cast<double *>(host), true);
+}
+
+int lbm_field_download(lbm_field f, double *host) { return transfer(f, host, false); }
+
+int lbm_field_copy(lbm_field dst, lbm_field src) {
+    if (int rc = check_field(dst, "lbm_field_copy")) return rc;
+    if (int rc = check_field(src, "lbm_field_copy")) return rc;
+    if (dst->bytes != src->bytes || dst->storage != src->storage || dst->qid != src->qid)
+        return fail(-1, "lbm_field_copy: fields differ in shape or storage");
+    if (dst == src) return 0;
+    if (int rc = lbm::dispatch_copy(src->d, dst->d, src->bytes, g_stream))
+        return fail(-4, "copy kernel failed (%d)", rc);
+    return 0;
+}
+
+// ---- models -----------------------------------------------------------------
+int lbm_model_create(lbm_model *out, int quadrature) {
+    if (!out) return fail(-1, "lbm_model_create: null output handle");
+    *out = nullptr;
+    HostQuad q;
+    if (!host_quad(quadrature, q)) return fail(-1, "unknown quadrature D2Q%d", quadrature);
+    if (int rc = ensure_stream()) return rc;
+    lbm_model m = new lbm_model_s;
+    m->qid = quadrature;
+    m->q = q;
+    // Hermite tables for MRT, orders 2..4 (mrt.jl:31)
+    const double cs = 1.0 / q.s;
+    std::vector<double> H((size_t)q.Q * (4 + 8 + 16), 0.0);
+    for (int n = 2; n <= 4; ++n) {
+        const int off = q.Q * (n == 2 ? 0 : (n == 3 ? 4 : 12));
+        const int mm = 1 << n;
+        for (int i = 0; i < q.Q; ++i) hermite_tensor(n, q.cx[i], q.cy[i], cs, &H[off + i * mm]);
+    }
+    hipError_t e = hipMalloc(&m->dH, H.size() * sizeof(double));
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(m->dH, H.data(), H.size() * sizeof(double), hipMemcpyHostToDevice,
+                           g_stream);
+    if (e == hipSuccess) e = hipMalloc(&m->dOut, 8 * sizeof(double));
+    if (e == hipSuccess) e = hipStreamSynchronize(g_stream);
+    if (e != hipSuccess) {
+        free_dev(reinterpret_cast<void *&>(m->dH));
+        delete m;
+        return fail(-3, "model allocation failed: %s", hipGetErrorString(e));
+    }
+    *out = m;
+    return 0;
+}
+
+int lbm_model_destroy(lbm_model m) {
+    if (!m) return 0;
+    if (g_stream) (void)hipStreamSynchronize(g_stream);
+    free_dev(reinterpret_cast<void *&>(m->dF));
+    free_dev(reinterpret_cast<void *&>(m->dB));
+    free_dev(reinterpret_cast<void *&>(m->dMw));
+    free_dev(reinterpret_cast<void *&>(m->dH));
+    free_dev(reinterpret_cast<void *&>(m->dPart));
+    free_dev(reinterpret_cast<void *&>(m->dOut));
+    delete m;
+    return 0;
+}
+
+int lbm_model_set_srt(lbm_model m, double tau) {
+    if (!m) return fail(-1, "null model");
+    if (!(tau > 0.0) || !std::isfinite(tau)) return fail(-1, "SRT tau must be positive");
+    m->coll = LBM_SRT;
+    m->tau = tau;
+    return 0;
+}
+
+int lbm_model_set_trt(lbm_model m, double tau_symmetric, double tau_asymmetric) {
+    if (!m) return fail(-1, "null model");
+    if (!(tau_symmetric > 0.0) || !(tau_asymmetric > 0.0))
+        return fail(-1, "TRT relaxation times must be positive");
+    m->coll = LBM_TRT;
+    m->tau_s = tau_symmetric;
+    m->tau_a = tau_asymmetric;
+    return 0;
+}
+
+int lbm_model_set_mrt(lbm_model m, const double *taus, int ntaus) {
+    if (!m) return fail(-1, "null model");
+    const int N = m->q.order / 2;
+    const int need = N > 2 ? N : 2;
+    if (!taus || ntaus < need)
+        return fail(-1, "MRT on D2Q%d needs at least %d relaxation times", m->qid, need);
+    for (int k = 0; k < ntaus; ++k)
+        if (!(taus[k] > 0.0)) return fail(-1, "MRT relaxation times must be positive");
+    m->coll = LBM_MRT;
+    m->taus.assign(taus, taus + ntaus);
+    return 0;
+}
+
+int lbm_model_set_force(lbm_model m, int kind, const double *values, int nx, int ny) {
+    if (!m) return fail(-1, "null model");
+    if (kind == LBM_FORCE_NONE) {
+        m->force_kind = kind;
+        return 0;
+    }
+    if (!values) return fail(-1, "force values are null");
+    if (kind == LBM_FORCE_CONST) {
+        m->force_kind = kind;
+        m->F[0] = values[0];
+        m->F[1] = values[1];
+        return 0;
+    }
+    if (kind != LBM_FORCE_FIELD) return fail(-1, "unknown force kind %d", kind);
+    if (nx < 1 || ny < 1) return fail(-1, "force field size must be positive");
+    const size_t bytes = (size_t)nx * ny * 2 * sizeof(double);
+    if (m->fnx != nx || m->fny != ny || !m->dF) {
+        free_dev(reinterpret_cast<void *&>(m->dF));
+        HIP_TRY(hipMalloc(&m->dF, bytes));
+    }
+    HIP_TRY(hipMemcpyAsync(m->dF, values, bytes, hipMemcpyHostToDevice, g_stream));
+    HIP_TRY(hipStreamSynchronize(g_stream));
+    m->fnx = nx;
+    m->fny = ny;
+    m->force_kind = kind;
+    return 0;
+}
+
+int lbm_model_set_bcs(lbm_model m, const lbm_bc *bcs, int n) {
+    if (!m) return fail(-1, "null model");
+    if (n < 0 || n > lbm::kMaxBC) return fail(-1, "at most %d boundary conditions", lbm::kMaxBC);
+    if (n > 0 && !bcs) return fail(-1, "null boundary-condition list");
+    std::vector<lbm::BCDev> dev(n > 0 ? n : 1);
+    std::vector<double> mw((size_t)(n > 0 ? n : 1) * m->q.Q, 0.0);
+    for (int b = 0; b < n; ++b) {
+        const lbm_bc &c = bcs[b];
+        if (c.kind != LBM_BOUNCE_BACK && c.kind != LBM_MOVING_WALL)
+            return fail(-1, "unknown boundary kind %d", c.kind);
+        if (c.direction < LBM_NORTH || c.direction > LBM_WEST)
+            return fail(-1, "unknown boundary direction %d", c.direction);
+        if (c.kind == LBM_MOVING_WALL && c.direction != LBM_NORTH)
+            return fail(-2, "MovingWall is defined for North only (moving_wall.jl:198)");
+        dev[b] = lbm::BCDev{c.kind, c.direction, c.x0 - 1, c.x1 - 1, c.y0 - 1, c.y1 - 1};
+        if (c.kind == LBM_MOVING_WALL) {
+            // a_1 = w_i * s * dot(rho_w u_w, H1(c_i)); f += 2 a_1 (moving_wall.jl:201-214)
+            const double a1eq[2] = {c.rho * c.ux, c.rho * c.uy};
+            for (int i = 0; i < m->q.Q; ++i) {
+                const double H1[2] = {(double)m->q.cx[i] * 1.0, (double)m->q.cy[i] * 1.0};
+                const double dot = a1eq[0] * H1[0] + a1eq[1] * H1[1];
+                const double a1 = (m->q.w[i] * m->q.s) * dot;
+                mw[(size_t)b * m->q.Q + i] = 2.0 * a1;
+            }
+        }
+    }
+    free_dev(reinterpret_cast<void *&>(m->dB));
+    free_dev(reinterpret_cast<void *&>(m->dMw));
+    HIP_TRY(hipMalloc(&m->dB, dev.size() * sizeof(lbm::BCDev)));
+    HIP_TRY(hipMalloc(&m->dMw, mw.size() * sizeof(double)));
+    HIP_TRY(hipMemcpyAsync(m->dB, dev.data(), dev.size() * sizeof(lbm::BCDev),
+                           hipMemcpyHostToDevice, g_stream));
+    HIP_TRY(hipMemcpyAsync(m->dMw, mw.data(), mw.size() * sizeof(double), hipMemcpyHostToDevice,
+                           g_stream));
+    HIP_TRY(hipStreamSynchronize(g_stream));
+    m->bcs.assign(bcs, bcs + n);
+    return 0;
+}
+
+// ---- operators --------------------------------------------------------------
+int lbm_collide(lbm_model m, lbm_field f_in, lbm_field f_out) {
+    if (int rc = check_pair(m, f_in, f_out, "lbm_collide", true)) return rc;
+    return run(lbm::Op::Collide, m, f_in, f_out, false);
+}
+
+int lbm_stream(lbm_model m, lbm_field f, lbm_field f_new) {
+    if (int rc = check_pair(m, f, f_new, "lbm_stream", false)) return rc;
+    return run(lbm::Op::StreamBC, m, f, f_new, false);
+}
+
+int lbm_apply_bcs(lbm_model m, lbm_field f_new, lbm_field f_old) {
+    if (int rc = check_pair(m, f_old, f_new, "lbm_apply_bcs", false)) return rc;
+    if (m->bcs.empty()) return 0;
+    return run(lbm::Op::ApplyBC, m, f_old, f_new, true);
+}
+
+int lbm_stream_bc(lbm_model m, lbm_field f, lbm_field f_new) {
+    if (int rc = check_pair(m, f, f_new, "lbm_stream_bc", false)) return rc;
+    return run(lbm::Op::StreamBC, m, f, f_new, true);
+}
+
+static int simulate_impl(lbm_model m, lbm_field S, lbm_field C, int64_t nsteps, double *ms,
+                         int64_t *nlaunch) {
+    if (int rc = check_pair(m, S, C, "lbm_simulate", true)) return rc;
+    if (nsteps < 0) return fail(-1, "negative step count");
+    if (ms) *ms = 0.0;
+    if (nlaunch) *nlaunch = 0;
+    if (nsteps == 0) return 0;
+    // step 1 starts from the canonical state S: C_0 = collide(S_0)
+    if (int rc = run(lbm::Op::Collide, m, S, C, false)) return rc;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (ms) {
+        HIP_TRY(hipEventCreate(&e0));
+        HIP_TRY(hipEventCreate(&e1));
+        HIP_TRY(hipEventRecord(e0, g_stream));
+    }
+    // steps 2..n: C_k = collide(BC(stream(C_{k-1}))) in one pull pass each
+    lbm_field cur = C, other = S;
+    for (int64_t k = 1; k < nsteps; ++k) {
+        if (int rc = run(lbm::Op::Fused, m, cur, other, true)) return rc;
+        std::swap(cur, other);
+    }
+    if (ms) {
+        HIP_TRY(hipEventRecord(e1, g_stream));
+    }
+    // S_n = BC(stream(C_{n-1}), C_{n-1})
+    if (int rc = run(lbm::Op::StreamBC, m, cur, other, true)) return rc;
+    if (cur != C) {
+        // the buffers ended in swapped roles: exchange the device arrays so the
+        // handles keep the reference's roles (f_stream holds S_n, f_collision C_{n-1})
+        std::swap(S->d, C->d);
+    }
+    if (ms) {
+        HIP_TRY(hipEventSynchronize(e1));
+        float t = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&t, e0, e1));
+        *ms = (double)t;
+        if (nlaunch) *nlaunch = nsteps - 1;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+    }
+    return 0;
+}
+
+int lbm_simulate(lbm_model m, lbm_field f_stream, lbm_field f_collision, int64_t nsteps) {
+    return simulate_impl(m, f_stream, f_collision, nsteps, nullptr, nullptr);
+}
+
+int lbm_simulate_timed(lbm_model m, lbm_field f_stream, lbm_field f_collision, int64_t nsteps,
+                       double *ms_total, int64_t *nlaunch) {
+    double ms = 0.0;
+    int64_t nl = 0;
+    int rc = simulate_impl(m, f_stream, f_collision, nsteps, &ms, &nl);
+    if (ms_total) *ms_total = ms;
+    if (nlaunch) *nlaunch = nl;
+    return rc;
+}
+
+// ---- moments ----------------------------------------------------------------
+int lbm_moments(lbm_field f, double tau, double *rho, double *ux, double *uy, double *sxx,
+                double *sxy, double *syy) {
+    if (int rc = check_field(f, "lbm_moments")) return rc;
+    HostQuad q;
+    host_quad(f->qid, q);
+    lbm::StepParams<double> p{};
+    p.nx = f->nx;
+    p.ny = f->ny;
+    p.plane = (long long)f->nx * f->ny;
+    for (int i = 0; i < 37; ++i) p.w[i] = i < q.Q ? q.w[i] : 0.0;
+    p.rho0 = f->rho0;
+    p.dscale = f->dscale;
+    p.dscale_inv = 1.0 / f->dscale;
+    p.cs_inv = 1.0 / q.s;
+    double *host[6] = {rho, ux, uy, sxx, sxy, syy};
+    double *dev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    const size_t nb = (size_t)p.plane * sizeof(double);
+    hipError_t e = hipSuccess;
+    for (int k = 0; k < 6 && e == hipSuccess; ++k)
+        if (host[k]) e = hipMalloc(&dev[k], nb);
+    int rc = 0;
+    if (e == hipSuccess)
+        rc = lbm::dispatch_moments(f->qid, f->storage, f->d, p, tau, dev[0], dev[1], dev[2],
+                                   dev[3], dev[4], dev[5], g_stream);
+    for (int k = 0; k < 6 && e == hipSuccess && !rc; ++k)
+        if (host[k]) e = hipMemcpyAsync(host[k], dev[k], nb, hipMemcpyDeviceToHost, g_stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(g_stream);
+    for (int k = 0; k < 6; ++k) free_dev(reinterpret_cast<void *&>(dev[k]));
+    if (e != hipSuccess) return fail(-3, "lbm_moments: %s", hipGetErrorString(e));
+    if (rc) return fail(-4, "moments kernel failed (%d)", rc);
+    return 0;
+}
+
+int lbm_reduce(lbm_field f, double *out5) {
+    if (int rc = check_field(f, "lbm_reduce")) return rc;
+    if (!out5) return fail(-1, "lbm_reduce: null output");
+    HostQuad q;
+    host_quad(f->qid, q);
+    lbm::StepParams<double> p{};
+    p.nx = f->nx;
+    p.ny = f->ny;
+    p.plane = (long long)f->nx * f->ny;
+    for (int i = 0; i < 37; ++i) p.w[i] = i < q.Q ? q.w[i] : 0.0;
+    p.rho0 = f->rho0;
+    p.dscale = f->dscale;
+    p.dscale_inv = 1.0 / f->dscale;
+    const int npart = (int)((p.plane + 255) / 256);
+    double *part = nullptr, *out = nullptr;
+    HIP_TRY(hipMalloc(&part, (size_t)npart * 5 * sizeof(double)));
+    hipError_t e = hipMalloc(&out, 8 * sizeof(double));
+    int rc = 0;
+    if (e == hipSuccess)
+        rc = lbm::dispatch_reduce(f->qid, f->storage, f->d, p, part, npart, out, g_stream);
+    if (e == hipSuccess && !rc)
+        e = hipMemcpyAsync(out5, out, 5 * sizeof(double), hipMemcpyDeviceToHost, g_stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(g_stream);
+    (void)hipFree(part);
+    if (out) (void)hipFree(out);
+    if (e != hipSuccess) return fail(-3, "lbm_reduce: %s", hipGetErrorString(e));
+    if (rc) return fail(-4, "reduction kernel failed (%d)", rc);
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,74 @@
+// lbm_internal.h -- types shared between the C-ABI host code and the kernel
+// translation units.  Not part of the public ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace lbm {
+
+constexpr int kMaxBC = 16;
+
+// One boundary condition with 0-based inclusive ranges.
+struct BCDev {
+    int kind;  // 0 bounce-back, 1 moving wall
+    int dir;   // 0 N, 1 E, 2 S, 3 W
+    int x0, x1, y0, y1;
+};
+
+// Per-launch parameters.  T is the arithmetic type (double for the plain
+// path, float for the reduced-precision storages).
+template <class T>
+struct StepParams {
+    int nx, ny;
+    long long plane;  // nx * ny
+    T w[37];
+    // equilibrium constants (reference: maxwell_boltzmann_equilibrium.jl:43-66,
+    // velocity_distribution_function/quadratures.jl:69-190)
+    T s, neg_s, s2, three_s, six_s3, three_s2, s4;
+    // SRT: omo = 1 - 1/tau, om = 1/tau; TRT: oms = 1/tau_s, oma = 1/tau_a
+    T omo, om, oms, oma;
+    T tau_force;  // tau (SRT), tau_a (TRT), taus[2] (MRT): velocity shift u + tau F
+    int force_kind;  // 0 none, 1 const, 2 field
+    T Fx, Fy;
+    const double *Ffield;  // (nx, ny, 2) column-major
+    int nbc;
+    const BCDev *bcs;  // device array [nbc]
+    const double *mw;  // device array [nbc * Q]: moving-wall term 2*a_1 per population
+    // reduced-precision storage: f_i = w_i * (rho0 + scale_i * g_i)
+    T rho0;
+    T dscale, dscale_inv;
+    // MRT (FP64 only)
+    int mrt_n;               // div(order, 2)
+    T mrt_omo[5], mrt_om[5]; // 1 - 1/taus[n], 1/taus[n] for n = 1..4 (index n)
+    T mrt_sn[5];             // s^n
+    const double *mrt_H;     // Hermite tables (see lbm_capi.cpp)
+    T cs_inv;                // 1/s
+};
+
+enum class Op : int { Collide = 0, StreamBC = 1, Fused = 2, ApplyBC = 3, Copy = 4, Convert = 5 };
+
+struct Launch {
+    Op op;
+    int qid;
+    int coll;     // 0 SRT, 1 TRT, 2 MRT
+    int storage;  // LBM_STORE_*
+    const void *src;
+    void *dst;
+    int block;    // threads per block along x (0 = default)
+};
+
+// Kernel dispatchers, one per translation unit (FP64 exact / reduced precision).
+int dispatch_f64(const Launch &L, const StepParams<double> &p, hipStream_t s);
+int dispatch_lp(const Launch &L, const StepParams<double> &p, hipStream_t s);
+// moments and reductions (FP64 accumulation for every storage)
+int dispatch_moments(int qid, int storage, const void *f, const StepParams<double> &p, double tau,
+                     double *rho, double *ux, double *uy, double *sxx, double *sxy, double *syy,
+                     hipStream_t s);
+int dispatch_reduce(int qid, int storage, const void *f, const StepParams<double> &p,
+                    double *partials, int npart, double *out5, hipStream_t s);
+int dispatch_upload(int qid, int storage, const double *src, void *dst,
+                    const StepParams<double> &p, int to_device, hipStream_t s);
+int dispatch_copy(const void *src, void *dst, size_t nbytes, hipStream_t s);
+
+}  // namespace lbm

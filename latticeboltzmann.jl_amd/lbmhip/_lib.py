@@ -1,0 +1,113 @@
+"""ctypes binding of liblbm_hip.so (the C ABI in include/lbm_hip.h).
+
+The library is the only compute path: if it is missing this module raises
+instead of falling back to anything else.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.environ.get("LBM_HIP_LIB",
+                          os.path.join(os.path.dirname(_HERE), "lib", "liblbm_hip.so"))
+
+STORE_F64, STORE_F32, STORE_F16DEV, STORE_BF16DEV = 0, 1, 2, 3
+STORAGES = {"f64": STORE_F64, "f32": STORE_F32, "f16dev": STORE_F16DEV, "bf16dev": STORE_BF16DEV}
+SRT_KIND, TRT_KIND, MRT_KIND = 0, 1, 2
+FORCE_NONE, FORCE_CONST, FORCE_FIELD = 0, 1, 2
+BOUNCE_BACK, MOVING_WALL = 0, 1
+NORTH, EAST, SOUTH, WEST = 0, 1, 2, 3
+
+# every entry point declared in include/lbm_hip.h
+EXPORTS = [
+    "lbm_last_error", "lbm_version", "lbm_device_count", "lbm_synchronize",
+    "lbm_quadrature_info",
+    "lbm_field_create", "lbm_field_destroy", "lbm_field_info", "lbm_field_upload",
+    "lbm_field_download", "lbm_field_copy",
+    "lbm_model_create", "lbm_model_destroy", "lbm_model_set_srt", "lbm_model_set_trt",
+    "lbm_model_set_mrt", "lbm_model_set_force", "lbm_model_set_bcs",
+    "lbm_collide", "lbm_stream", "lbm_apply_bcs", "lbm_stream_bc", "lbm_simulate",
+    "lbm_simulate_timed", "lbm_moments", "lbm_reduce",
+]
+
+
+class LbmBC(ctypes.Structure):
+    _fields_ = [
+        ("kind", ctypes.c_int32),
+        ("direction", ctypes.c_int32),
+        ("x0", ctypes.c_int32),
+        ("x1", ctypes.c_int32),
+        ("y0", ctypes.c_int32),
+        ("y1", ctypes.c_int32),
+        ("ux", ctypes.c_double),
+        ("uy", ctypes.c_double),
+        ("rho", ctypes.c_double),
+    ]
+
+
+class LbmError(RuntimeError):
+    """An lbm_* call returned a negative status."""
+
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is missing: build it with `python latticeboltzmann.jl_amd/build.py` "
+            "(there is no CPU fallback)")
+    L = ctypes.CDLL(LIB_PATH)
+    vp = ctypes.c_void_p
+    dp = ctypes.POINTER(ctypes.c_double)
+    ip = ctypes.POINTER(ctypes.c_int)
+    i = ctypes.c_int
+    d = ctypes.c_double
+    i64 = ctypes.c_int64
+    L.lbm_last_error.restype = ctypes.c_char_p
+    L.lbm_last_error.argtypes = []
+    L.lbm_version.restype = i
+    L.lbm_device_count.restype = i
+    L.lbm_synchronize.restype = i
+    L.lbm_quadrature_info.argtypes = [i, ip, dp, dp, dp, dp, ip, ip]
+    L.lbm_field_create.argtypes = [ctypes.POINTER(vp), i, i, i, i, d]
+    L.lbm_field_destroy.argtypes = [vp]
+    L.lbm_field_info.argtypes = [vp, ip, ip, ip, ip, ip]
+    L.lbm_field_upload.argtypes = [vp, dp]
+    L.lbm_field_download.argtypes = [vp, dp]
+    L.lbm_field_copy.argtypes = [vp, vp]
+    L.lbm_model_create.argtypes = [ctypes.POINTER(vp), i]
+    L.lbm_model_destroy.argtypes = [vp]
+    L.lbm_model_set_srt.argtypes = [vp, d]
+    L.lbm_model_set_trt.argtypes = [vp, d, d]
+    L.lbm_model_set_mrt.argtypes = [vp, dp, i]
+    L.lbm_model_set_force.argtypes = [vp, i, dp, i, i]
+    L.lbm_model_set_bcs.argtypes = [vp, ctypes.POINTER(LbmBC), i]
+    for name in ["lbm_collide", "lbm_stream", "lbm_apply_bcs", "lbm_stream_bc"]:
+        getattr(L, name).argtypes = [vp, vp, vp]
+    L.lbm_simulate.argtypes = [vp, vp, vp, i64]
+    L.lbm_simulate_timed.argtypes = [vp, vp, vp, i64, dp, ctypes.POINTER(i64)]
+    L.lbm_moments.argtypes = [vp, d, dp, dp, dp, dp, dp, dp]
+    L.lbm_reduce.argtypes = [vp, dp]
+    for name in EXPORTS:
+        f = getattr(L, name)
+        if name not in ("lbm_last_error",):
+            f.restype = ctypes.c_int
+    _lib = L
+    return L
+
+
+def check(rc: int, what: str = "") -> None:
+    if rc < 0:
+        msg = lib().lbm_last_error().decode(errors="replace")
+        raise LbmError(f"{what}: {msg} (status {rc})" if what else f"{msg} (status {rc})")
+
+
+def dptr(a):
+    if a is None:
+        return None
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))

@@ -1,0 +1,277 @@
+"""GPU parity: the gfx950 kernels (through the C ABI) against the CPU oracle.
+
+Bar: the plain FP64 path is bit-exact against the oracle (same evaluation
+order, no FMA contraction).  The north-star tolerance for the FP64 path is
+1e-12 relative per population after N steps; asserting equality is stricter.
+"""
+import numpy as np
+import pytest
+
+from lbm_cases import (BC_NAMES, QIDS, bc_dicts, force_for, random_populations,
+                       shear_wave_fields, taylor_green_vortex_pops, to_lbmhip)
+
+pytestmark = pytest.mark.gpu
+
+
+def _L():
+    import lbmhip
+    return lbmhip
+
+
+def _q(qid):
+    L = _L()
+    return L.Quadratures[f"D2Q{qid}"]()
+
+
+def _cm(kind, qid, force=None):
+    L = _L()
+    if kind == 0:
+        return L.SRT(0.8, force), [0.8]
+    if kind == 1:
+        return L.TRT.fields(0.7, 1.3, force), [0.7, 1.3]
+    q = _q(qid)
+    N = max(2, L.order(q) // 2)
+    taus = [0.9, 0.75, 1.1, 0.6][:max(N, 2)]
+    return L.MRT(q, taus, force=force), taus
+
+
+def _assert_same(a, b, what):
+    if not np.array_equal(a, b):
+        d = np.abs(a - b)
+        rel = d.max() / np.abs(b).max()
+        raise AssertionError(f"{what}: not bit-exact, max |diff| {d.max():.3e} (rel {rel:.3e}), "
+                             f"{np.count_nonzero(d)} of {d.size} differ")
+
+
+@pytest.mark.parametrize("qid", QIDS)
+@pytest.mark.parametrize("coll", [0, 1, 2])
+@pytest.mark.parametrize("force", ["none", "const", "field"])
+def test_collide_bit_exact(oracle_lib, qid, coll, force):
+    L = _L()
+    nx, ny = 13, 7
+    f = random_populations(qid, nx, ny, seed=qid + 10 * coll)
+    F = force_for(force, nx, ny)
+    cm, taus = _cm(coll, qid, F)
+    q = _q(qid)
+    fin = L.HipLattice.from_numpy(q, f)
+    fout = L.HipLattice(q, nx, ny)
+    L.collide_(cm, q, fin, fout, time=0.0)
+    got = fout.to_numpy()
+    ref = np.zeros_like(f, order="F")
+    if coll == 0:
+        oracle_lib.collide_srt(qid, f, ref, taus[0], F)
+    elif coll == 1:
+        oracle_lib.collide_trt(qid, f, ref, taus[0], taus[1], F)
+    else:
+        oracle_lib.collide_mrt(qid, f, ref, taus, F)
+    _assert_same(got, ref, f"collide D2Q{qid} coll={coll} force={force}")
+    # the input is left untouched
+    _assert_same(fin.to_numpy(), f, "collide input")
+
+
+@pytest.mark.parametrize("qid", QIDS)
+@pytest.mark.parametrize("shape", [(11, 6), (1, 1), (2, 3), (1, 5)])
+def test_stream_bit_exact(oracle_lib, qid, shape):
+    L = _L()
+    nx, ny = shape
+    f = random_populations(qid, nx, ny, seed=3)
+    q = _q(qid)
+    a = L.HipLattice.from_numpy(q, f)
+    b = L.HipLattice(q, nx, ny)
+    L.stream_(q, a, b)
+    ref = np.zeros_like(f, order="F")
+    oracle_lib.stream(qid, f, ref)
+    _assert_same(b.to_numpy(), ref, f"stream D2Q{qid} {shape}")
+
+
+@pytest.mark.parametrize("qid", QIDS)
+@pytest.mark.parametrize("bcname", BC_NAMES[1:])
+def test_apply_bcs_bit_exact(oracle_lib, qid, bcname):
+    L = _L()
+    nx, ny = 9, 8
+    fnew = random_populations(qid, nx, ny, seed=4)
+    fold = random_populations(qid, nx, ny, seed=5)
+    bcs = bc_dicts(bcname, nx, ny)
+    q = _q(qid)
+    dn = L.HipLattice.from_numpy(q, fnew)
+    do = L.HipLattice.from_numpy(q, fold)
+    L.apply_(to_lbmhip(bcs), q, dn, do)
+    ref = fnew.copy(order="F")
+    oracle_lib.apply_bcs(qid, bcs, ref, fold)
+    _assert_same(dn.to_numpy(), ref, f"apply! D2Q{qid} {bcname}")
+
+
+@pytest.mark.parametrize("qid", QIDS)
+@pytest.mark.parametrize("coll", [0, 1, 2])
+@pytest.mark.parametrize("bcname", BC_NAMES)
+def test_simulate_bit_exact(oracle_lib, qid, coll, bcname):
+    """Fused pull kernel: both model buffers equal the reference step loop."""
+    L = _L()
+    nx, ny = 12, 10
+    f0 = random_populations(qid, nx, ny, seed=7)
+    F = force_for("const" if bcname == "poiseuille" else "none", nx, ny)
+    cm, taus = _cm(coll, qid, F)
+    bcs = bc_dicts(bcname, nx, ny)
+    q = _q(qid)
+    model = L.LatticeBoltzmannModel.from_populations(q, f0, cm, to_lbmhip(bcs))
+    nsteps = 17
+    L.simulate(model, range(nsteps))
+    S = f0.copy(order="F")
+    C = f0.copy(order="F")
+    oracle_lib.simulate(qid, S, C, coll, taus, nsteps, force=F, bcs=bcs)
+    _assert_same(model.f_stream.to_numpy(), S, f"f_stream D2Q{qid} coll={coll} {bcname}")
+    _assert_same(model.f_collision.to_numpy(), C, f"f_collision D2Q{qid} coll={coll} {bcname}")
+
+
+@pytest.mark.parametrize("qid", [9, 17, 37])
+@pytest.mark.parametrize("shape", [(1, 5), (3, 5), (2, 2), (1, 1), (5, 3)])
+def test_tiny_lattices(oracle_lib, qid, shape):
+    """|c| >= n wraps (stream.jl:69-74) and BCs hitting most rows (nx = 1 as
+    CouetteFlow, couette_flow.jl:115; NX = 3 as PoiseuilleFlow, poiseuille.jl:22)."""
+    L = _L()
+    nx, ny = shape
+    f0 = random_populations(qid, nx, ny, seed=11)
+    for bcname in ["periodic", "couette", "poiseuille"]:
+        bcs = bc_dicts(bcname, nx, ny)
+        cm, taus = _cm(1, qid)
+        q = _q(qid)
+        model = L.LatticeBoltzmannModel.from_populations(q, f0, cm, to_lbmhip(bcs))
+        L.simulate(model, range(6))
+        S = f0.copy(order="F")
+        C = f0.copy(order="F")
+        oracle_lib.simulate(qid, S, C, 1, taus, 6, bcs=bcs)
+        _assert_same(model.f_stream.to_numpy(), S, f"tiny {shape} D2Q{qid} {bcname}")
+
+
+def test_separate_operators_match_fused(oracle_lib):
+    """collide_ + stream_ + apply_ (the model-level calls) == simulate."""
+    L = _L()
+    qid, nx, ny = 9, 16, 12
+    f0 = random_populations(qid, nx, ny, seed=2)
+    bcs = to_lbmhip(bc_dicts("couette", nx, ny))
+    q = _q(qid)
+    a = L.LatticeBoltzmannModel.from_populations(q, f0, L.SRT(0.9), bcs)
+    b = L.LatticeBoltzmannModel.from_populations(q, f0, L.SRT(0.9), bcs)
+    for t in range(5):
+        L.collide_model_(a, time=float(t))
+        L.stream_model_(a)
+        L.apply_boundary_conditions_(a, time=float(t))
+    L.simulate(b, range(5))
+    _assert_same(a.f_stream.to_numpy(), b.f_stream.to_numpy(), "separate vs fused f_stream")
+    _assert_same(a.f_collision.to_numpy(), b.f_collision.to_numpy(), "separate vs fused f_coll")
+
+
+def test_callable_force_per_step(oracle_lib):
+    """A time-dependent force closure (x_idx, y_idx, t) is re-evaluated each step."""
+    L = _L()
+    qid, nx, ny = 9, 8, 6
+    f0 = random_populations(qid, nx, ny, seed=9)
+
+    def force(x, y, t):
+        return (1e-5 * np.cos(0.3 * y - 0.1 * t), 2e-6 * np.sin(0.2 * x))
+
+    q = _q(qid)
+    model = L.LatticeBoltzmannModel.from_populations(q, f0, L.SRT(0.8, force), delta_t=1.0)
+    L.simulate(model, range(4))
+    S = f0.copy(order="F")
+    C = f0.copy(order="F")
+    for t in range(4):
+        F = np.empty((nx, ny, 2), order="F")
+        for x in range(nx):
+            for y in range(ny):
+                F[x, y] = force(x + 1, y + 1, float(t))
+        oracle_lib.simulate(qid, S, C, 0, [0.8], 1, force=F)
+    _assert_same(model.f_stream.to_numpy(), S, "time-dependent force")
+
+
+def test_shear_wave_c1(oracle_lib):
+    """C1: D2Q9 SRT, periodic 64x64, tau = 0.8, u_x = 0.01 sin(2 pi y/64),
+    equilibrium init, 1000 steps.  GPU == oracle, and the decay of the mode
+    matches exp(-nu k^2 t), nu = (tau - 1/2)/3, to 1e-3."""
+    L = _L()
+    n, tau, steps = 64, 0.8, 1000
+    q = L.D2Q9()
+    rho, ux, uy = shear_wave_fields(n)
+    f0 = L.equilibrium_populations(q, rho, ux, uy)
+    model = L.LatticeBoltzmannModel.from_populations(q, f0, L.SRT(tau))
+    L.simulate(model, range(steps))
+    S = f0.copy(order="F")
+    C = f0.copy(order="F")
+    oracle_lib.simulate(9, S, C, 0, [tau], steps)
+    got = model.f_stream.to_numpy()
+    rel = np.abs(got - S).max() / np.abs(S).max()
+    assert rel <= 1e-12, rel
+    _assert_same(got, S, "C1 shear wave")
+    m = L.moments(model.f_stream, tau, stress=False)
+    y = np.arange(n)
+    amp = 2.0 / n * np.sum(m.ux[0] * np.sin(2 * np.pi * y / n))
+    nu = (tau - 0.5) / 3.0
+    k = 2 * np.pi / n
+    expect = 0.01 * np.exp(-nu * k * k * steps)
+    # the LBM shear mode decays at nu k^2 (1 + O(k^2)); k = 2 pi / 64 gives ~1e-3
+    assert abs(amp / expect - 1.0) < 2e-3, (amp, expect)
+
+
+def test_taylor_green_trt_256(oracle_lib):
+    """C2 at N = 256 (fewer steps): D2Q9 TRT Lambda = 1/4, tau = 0.6."""
+    L = _L()
+    n, tau, steps = 256, 0.6, 200
+    q = L.D2Q9()
+    f0 = taylor_green_vortex_pops(q, n)
+    cm = L.trt_magic(tau, 0.25)
+    model = L.LatticeBoltzmannModel.from_populations(q, f0, cm)
+    L.simulate(model, range(steps))
+    S = f0.copy(order="F")
+    C = f0.copy(order="F")
+    oracle_lib.set_threads(8)
+    oracle_lib.simulate(9, S, C, 1, [cm.τ_symmetric, cm.τ_asymmetric], steps)
+    oracle_lib.set_threads(1)
+    _assert_same(model.f_stream.to_numpy(), S, "TGV 256 TRT")
+
+
+@pytest.mark.parametrize("qid", QIDS)
+def test_moments_and_sums(oracle_lib, qid):
+    L = _L()
+    nx, ny = 19, 7
+    f = random_populations(qid, nx, ny, seed=21)
+    q = _q(qid)
+    d = L.HipLattice.from_numpy(q, f)
+    m = L.moments(d, 0.7)
+    ref = oracle_lib.moments(qid, f, 0.7)
+    for a, b, name in zip(m, ref, ["rho", "ux", "uy", "sxx", "sxy", "syy"]):
+        _assert_same(a, b, f"moments {name} D2Q{qid}")
+    s = L.conserved_sums(d)
+    rho, ux, uy = ref[0], ref[1], ref[2]
+    np.testing.assert_allclose(s.mass, rho.sum(), rtol=1e-14)
+    np.testing.assert_allclose(s.momentum_x, (rho * ux).sum(), rtol=1e-12, atol=1e-15)
+    np.testing.assert_allclose(s.momentum_y, (rho * uy).sum(), rtol=1e-12, atol=1e-15)
+    np.testing.assert_allclose(s.energy, (rho * (ux * ux + uy * uy)).sum(), rtol=1e-12)
+    assert s.nonfinite_nodes == 0
+
+
+def test_nonfinite_detection():
+    """A NaN written from the host is counted by the reduction (the NaN stop
+    criterion of stopping_criteria.jl:47-50)."""
+    L = _L()
+    q = L.D2Q9()
+    f = random_populations(9, 8, 8, seed=1)
+    f[3, 4, 2] = np.nan
+    f[5, 1, 0] = np.inf
+    d = L.HipLattice.from_numpy(q, f)
+    assert L.conserved_sums(d).nonfinite_nodes == 2
+
+
+def test_abi_errors():
+    L = _L()
+    q = L.D2Q9()
+    a = L.HipLattice(q, 8, 8)
+    with pytest.raises(L.LbmError, match="distinct"):
+        L.collide_(L.SRT(0.8), q, a, a)
+    b = L.HipLattice(L.D2Q17(), 8, 8)
+    with pytest.raises(ValueError, match="populations"):
+        L.stream_(q, a, b)
+    c = L.HipLattice(q, 8, 8, storage="f32")
+    with pytest.raises(L.LbmError):
+        L.stream_(q, a, c)
+    with pytest.raises(L.LbmError, match="outside"):
+        L.apply_([L.BounceBack(L.North(), (1, 9), (1, 8))], q, a, L.HipLattice(q, 8, 8))
