@@ -33,8 +33,17 @@ enum { LBM_D2Q4 = 4, LBM_D2Q5 = 5, LBM_D2Q9 = 9, LBM_D2Q13 = 13,
        LBM_D2Q17 = 17, LBM_D2Q21 = 21, LBM_D2Q37 = 37 };
 
 /* population storage in HBM.  F64 is the plain (reference-exact) form; the
- * others are the reduced-precision forms with FP32 collision arithmetic. */
-enum { LBM_STORE_F64 = 0, LBM_STORE_F32 = 1, LBM_STORE_F16DEV = 2, LBM_STORE_BF16DEV = 3 };
+ * others are reduced-precision forms with FP32 collision arithmetic:
+ *   F32       populations in binary32;
+ *   F16DEV    deviations f_i - w_i rho0 in binary16;
+ *   BF16DEV   the same in bfloat16;
+ *   F16NEQ    per node a 16-byte record of FP32 moments (rho - rho0, ux, uy)
+ *             and, per population, the non-equilibrium part f_i - feq_i(rho, u)
+ *             in binary16 (scaled by 2^16);
+ *   BF16NEQ   the same in bfloat16 (unscaled).
+ * F32 / *DEV / *NEQ arithmetic runs on g_i = f_i - w_i rho0 (rho0 of the field). */
+enum { LBM_STORE_F64 = 0, LBM_STORE_F32 = 1, LBM_STORE_F16DEV = 2, LBM_STORE_BF16DEV = 3,
+       LBM_STORE_F16NEQ = 4, LBM_STORE_BF16NEQ = 5 };
 
 /* collision models: src/collision_models/{srt,trt,mrt}.jl */
 enum { LBM_SRT = 0, LBM_TRT = 1, LBM_MRT = 2 };
@@ -101,6 +110,17 @@ int lbm_model_set_mrt(lbm_model m, const double *taus, int ntaus);
 int lbm_model_set_force(lbm_model m, int kind, const double *values, int nx, int ny);
 /* ordered list, applied in order (boundary_conditions.jl:80-90) */
 int lbm_model_set_bcs(lbm_model m, const lbm_bc *bcs, int n);
+
+/* Launch shape of the fused step kernel: threads per block along x (64, 128,
+ * 256), lattice nodes per thread along x (1, 2, 4, 8) and non-temporal
+ * stores (0/1); 0 (or -1 for nontemporal) = automatic.  No counterpart in the
+ * reference (a CPU loop has no launch shape). */
+int lbm_model_set_launch(lbm_model m, int block_x, int nodes_per_thread, int nontemporal);
+/* Autotuner: times every launch shape of the fused step for this model on
+ * scratch copies of f_stream (the caller's fields are not advanced), keeps
+ * the fastest in the model and returns it.  No counterpart in the reference. */
+int lbm_autotune(lbm_model m, lbm_field f_stream, int trial_steps, int *block_x,
+                 int *nodes_per_thread, int *nontemporal, double *us_per_step);
 
 /* ---- step operators ----------------------------------------------------- */
 /* collide!(cm, q, f_in, f_out; time) -- collision_models.jl:19, srt.jl:37-81,

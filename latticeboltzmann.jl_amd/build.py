@@ -5,12 +5,15 @@
 
 Translation units and their floating-point contracts:
 
-* ``kernels_f64.hip``, ``kernels_aux.hip``, ``lbm_capi.cpp``:
+* ``k_f64_*.hip``, ``kernels_aux.hip``, ``lbm_capi.cpp``, ``lbm_dispatch.cpp``:
   ``-ffp-contract=off`` -- every multiply and add rounds separately, the
   reference's Julia semantics, so the FP64 path reproduces the CPU restatement
   bit for bit;
-* ``kernels_lp.hip``: reduced-precision storages with FP32 arithmetic,
-  ``-ffp-contract=fast``.
+* ``k_{f32,f16dev,bf16dev}_*.hip``: reduced-precision storages with FP32
+  arithmetic, ``-ffp-contract=fast``.
+
+One translation unit per (storage, quadrature group) keeps each hipcc job
+small; they compile in parallel.
 
 All device code is built with packed FP32 instructions disabled
 (``-target-feature -packed-fp32-ops``).
@@ -40,12 +43,15 @@ COMMON = [
     f"-I{os.path.join(REPO, 'include')}", f"-I{CSRC}",
 ]
 
-UNITS = [
-    ("kernels_f64.hip", ["-ffp-contract=off"]),
-    ("kernels_aux.hip", ["-ffp-contract=off"]),
-    ("kernels_lp.hip", ["-ffp-contract=fast"]),
-    ("lbm_capi.cpp", ["-x", "hip", "-ffp-contract=off"]),
-]
+def _units():
+    units = [("kernels_aux.hip", ["-ffp-contract=off"]),
+             ("lbm_capi.cpp", ["-x", "hip", "-ffp-contract=off"]),
+             ("lbm_dispatch.cpp", ["-x", "hip", "-ffp-contract=off"])]
+    for name in sorted(os.listdir(CSRC)):
+        if name.startswith("k_") and name.endswith(".hip"):
+            contract = "off" if name.startswith("k_f64_") else "fast"
+            units.append((name, [f"-ffp-contract={contract}"]))
+    return units
 
 HEADERS = ["lbm_quad.h", "lbm_internal.h", "lbm_kernels.h"]
 
@@ -59,20 +65,29 @@ def build(force: bool = False, verbose: bool = False) -> str:
     os.makedirs(LIBDIR, exist_ok=True)
     deps = [os.path.join(CSRC, h) for h in HEADERS] + [os.path.join(REPO, "include", "lbm_hip.h")]
     newest_dep = max(_mtime(d) for d in deps)
-    procs = []
+    jobs = []
     objs = []
-    for src, flags in UNITS:
+    for src, flags in _units():
         s = os.path.join(CSRC, src)
         o = os.path.join(BUILD, src.rsplit(".", 1)[0] + ".o")
         objs.append(o)
         if not force and _mtime(o) >= max(_mtime(s), newest_dep):
             continue
-        cmd = [HIPCC, *COMMON, *flags, "-c", s, "-o", o]
-        if verbose:
-            print(" ".join(cmd), flush=True)
-        procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)))
+        jobs.append((src, [HIPCC, *COMMON, *flags, "-c", s, "-o", o]))
+    # heaviest units first, at most one hipcc per CPU
+    jobs.sort(key=lambda j: (not j[0].startswith("k_"), "q37" not in j[0], j[0]))
+    width = max(1, min(len(jobs), os.cpu_count() or 1, 16))
+    procs = []
     failed = []
-    for src, p in procs:
+    pending = list(jobs)
+    while pending or procs:
+        while pending and len(procs) < width:
+            src, cmd = pending.pop(0)
+            if verbose:
+                print(" ".join(cmd), flush=True)
+            procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE,
+                                                stderr=subprocess.STDOUT)))
+        src, p = procs.pop(0)
         out, _ = p.communicate()
         text = out.decode(errors="replace")
         # the host pass does not know the packed-fp32 target feature; that warning is harmless

@@ -1,0 +1,8 @@
+// k_bf16dev_q13_17.hip -- fused / collide / stream kernels, storage bf16dev, D2Q{13,17}.
+// FP32 arithmetic on reduced storage: compiled with -ffp-contract=fast.
+#include "lbm_kernels.h"
+
+namespace lbm {
+LBM_DEFINE_RUN(bf16dev, StBF16Dev, 13)
+LBM_DEFINE_RUN(bf16dev, StBF16Dev, 17)
+}  // namespace lbm

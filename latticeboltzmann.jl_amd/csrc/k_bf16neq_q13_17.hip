@@ -1,0 +1,8 @@
+// k_bf16neq_q13_17.hip -- fused / collide / stream kernels, storage bf16neq, D2Q{13,17}.
+// FP32 arithmetic on reduced storage: compiled with -ffp-contract=fast.
+#include "lbm_kernels.h"
+
+namespace lbm {
+LBM_DEFINE_RUN(bf16neq, StBF16Neq, 13)
+LBM_DEFINE_RUN(bf16neq, StBF16Neq, 17)
+}  // namespace lbm

@@ -1,0 +1,8 @@
+// k_f32_q13_17.hip -- fused / collide / stream kernels, storage f32, D2Q{13,17}.
+// FP32 arithmetic on reduced storage: compiled with -ffp-contract=fast.
+#include "lbm_kernels.h"
+
+namespace lbm {
+LBM_DEFINE_RUN(f32, StF32, 13)
+LBM_DEFINE_RUN(f32, StF32, 17)
+}  // namespace lbm

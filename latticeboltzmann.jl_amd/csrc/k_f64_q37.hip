@@ -1,0 +1,7 @@
+// k_f64_q37.hip -- fused / collide / stream kernels, storage f64, D2Q{37}.
+// Plain FP64: compiled with -ffp-contract=off (bit-exact with the CPU restatement).
+#include "lbm_kernels.h"
+
+namespace lbm {
+LBM_DEFINE_RUN(f64, StF64, 37)
+}  // namespace lbm

@@ -5,9 +5,16 @@
 #include <hip/hip_bf16.h>
 
 #include "lbm_internal.h"
+#include "lbm_kernels.h"
 #include "lbm_quad.h"
 
 namespace lbm {
+
+// The FP32-arithmetic storages hold shifted populations g_i = f_i - w_i rho0
+// with the weights and rho0 the kernels use (binary32 values).
+__device__ __forceinline__ double shift_of(const StepParams<double> &p, int i) {
+    return (double)(float)p.w[i] * (double)(float)p.rho0;
+}
 
 // population i at linear node n, reconstructed in FP64 from any storage
 template <int SID>
@@ -17,14 +24,15 @@ __device__ __forceinline__ double load_d(const void *base, long long plane, long
     if constexpr (SID == 0) {
         return static_cast<const double *>(base)[k];
     } else if constexpr (SID == 1) {
-        return static_cast<double>(static_cast<const float *>(base)[k]);
+        const double g = static_cast<double>(static_cast<const float *>(base)[k]);
+        return shift_of(p, i) + g;
     } else if constexpr (SID == 2) {
         const double g = static_cast<double>(__half2float(static_cast<const __half *>(base)[k]));
-        return p.w[i] * p.rho0 + g * p.dscale_inv;
+        return shift_of(p, i) + g * p.dscale_inv;
     } else {
         const double g =
             static_cast<double>(__bfloat162float(static_cast<const __hip_bfloat16 *>(base)[k]));
-        return p.w[i] * p.rho0 + g * p.dscale_inv;
+        return shift_of(p, i) + g * p.dscale_inv;
     }
 }
 
@@ -35,12 +43,12 @@ __device__ __forceinline__ void store_d(void *base, long long plane, long long n
     if constexpr (SID == 0) {
         static_cast<double *>(base)[k] = v;
     } else if constexpr (SID == 1) {
-        static_cast<float *>(base)[k] = static_cast<float>(v);
+        static_cast<float *>(base)[k] = static_cast<float>(v - shift_of(p, i));
     } else if constexpr (SID == 2) {
-        const double g = (v - p.w[i] * p.rho0) * p.dscale;
+        const double g = (v - shift_of(p, i)) * p.dscale;
         static_cast<__half *>(base)[k] = __float2half_rn(static_cast<float>(g));
     } else {
-        const double g = (v - p.w[i] * p.rho0) * p.dscale;
+        const double g = (v - shift_of(p, i)) * p.dscale;
         static_cast<__hip_bfloat16 *>(base)[k] = __float2bfloat16(static_cast<float>(g));
     }
 }
@@ -252,8 +260,78 @@ int dispatch_reduce(int qid, int storage, const void *f, const StepParams<double
     return hipGetLastError() == hipSuccess ? 0 : -4;
 }
 
+// Float64 (nx, ny, Q) <-> non-equilibrium storage (moments in FP32, feq in FP32
+// with the kernels' formula, 16-bit non-equilibrium parts)
+template <int QID, class ST>
+__global__ void __launch_bounds__(256)
+k_neq_codec(double *__restrict__ hostlike, typename ST::store_t *dev, int to_device,
+            const StepParams<float> pf) {
+    constexpr int Q = QT<QID>::Q;
+    const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= pf.plane) return;
+    Mom4 *m = ST::mom(dev);
+    auto *P = ST::pops(dev, pf.plane);
+    if (to_device) {
+        double f[Q];
+#pragma unroll
+        for (int i = 0; i < Q; ++i) f[i] = hostlike[pf.plane * i + n];
+        // shifted populations g_i = f_i - w_i rho0 in FP64, moments from them
+        double drho = 0.0, mx = 0.0, my = 0.0, g[Q];
+#pragma unroll
+        for (int i = 0; i < Q; ++i) {
+            g[i] = f[i] - (double)pf.w[i] * (double)pf.rho0;
+            drho += g[i];
+            mx += g[i] * QT<QID>::cx(i);
+            my += g[i] * QT<QID>::cy(i);
+        }
+        const double rho = (double)pf.rho0 + drho;
+        const float r = (float)drho, ux = (float)(mx / rho), uy = (float)(my / rho);
+        float geq[Q];
+        feq_dev_all<QID, float>(r, ux, uy, pf, geq);
+        m[n] = Mom4{r, ux, uy, 0.0f};
+#pragma unroll
+        for (int i = 0; i < Q; ++i)
+            P[pf.plane * i + n] = ST::from_f((float)(g[i] - (double)geq[i]), pf);
+    } else {
+        float geq[Q];
+        const Mom4 mn = m[n];
+        feq_dev_all<QID, float>(mn.drho, mn.ux, mn.uy, pf, geq);
+#pragma unroll
+        for (int i = 0; i < Q; ++i)
+            hostlike[pf.plane * i + n] = (double)pf.w[i] * (double)pf.rho0 + (double)geq[i] +
+                                         (double)ST::to_f(P[pf.plane * i + n], pf);
+    }
+}
+
+template <int QID>
+static int neq_codec_q(int storage, double *tmp, void *dst, const StepParams<float> &pf,
+                       int to_device, hipStream_t s) {
+    const int blocks = (int)((pf.plane + 255) / 256);
+    if (storage == 4)
+        k_neq_codec<QID, StF16Neq><<<blocks, 256, 0, s>>>(tmp, static_cast<__half *>(dst),
+                                                           to_device, pf);
+    else
+        k_neq_codec<QID, StBF16Neq><<<blocks, 256, 0, s>>>(
+            tmp, static_cast<__hip_bfloat16 *>(dst), to_device, pf);
+    return hipGetLastError() == hipSuccess ? 0 : -4;
+}
+
 int dispatch_upload(int qid, int storage, const double *src, void *dst,
                     const StepParams<double> &p, int to_device, hipStream_t s) {
+    if (storage == 4 || storage == 5) {
+        const StepParams<float> pf = to_float_params(p);
+        double *tmp = const_cast<double *>(src);
+        switch (qid) {
+        case 4: return neq_codec_q<4>(storage, tmp, dst, pf, to_device, s);
+        case 5: return neq_codec_q<5>(storage, tmp, dst, pf, to_device, s);
+        case 9: return neq_codec_q<9>(storage, tmp, dst, pf, to_device, s);
+        case 13: return neq_codec_q<13>(storage, tmp, dst, pf, to_device, s);
+        case 17: return neq_codec_q<17>(storage, tmp, dst, pf, to_device, s);
+        case 21: return neq_codec_q<21>(storage, tmp, dst, pf, to_device, s);
+        case 37: return neq_codec_q<37>(storage, tmp, dst, pf, to_device, s);
+        default: return -1;
+        }
+    }
     int Q = qid == 21 ? 25 : qid;
     const long long nodes = p.plane;
     const int blocks = (int)((nodes + 255) / 256);

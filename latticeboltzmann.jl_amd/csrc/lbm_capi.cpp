@@ -155,7 +155,9 @@ int storage_bytes(int storage) {
     case LBM_STORE_F64: return 8;
     case LBM_STORE_F32: return 4;
     case LBM_STORE_F16DEV:
-    case LBM_STORE_BF16DEV: return 2;
+    case LBM_STORE_BF16DEV:
+    case LBM_STORE_F16NEQ:
+    case LBM_STORE_BF16NEQ: return 2;
     default: return 0;
     }
 }
@@ -215,6 +217,8 @@ struct lbm_model_s {
     double *dPart = nullptr;  // reduction partials
     size_t npart = 0;
     double *dOut = nullptr;
+    // launch shape of the fused step (0 = automatic)
+    int launch_block = 0, launch_vec = 0, launch_nt = -1;
 };
 
 namespace {
@@ -224,7 +228,8 @@ int check_field(lbm_field f, const char *what) {
     return 0;
 }
 
-int check_pair(lbm_model m, lbm_field a, lbm_field b, const char *what, bool uses_force) {
+int check_pair(lbm_model m, lbm_field a, lbm_field b, const char *what, bool uses_force,
+               bool uses_bcs) {
     if (!m) return fail(-1, "%s: null model", what);
     if (int rc = check_field(a, what)) return rc;
     if (int rc = check_field(b, what)) return rc;
@@ -239,6 +244,7 @@ int check_pair(lbm_model m, lbm_field a, lbm_field b, const char *what, bool use
         return fail(-1, "%s: force field is %dx%d but the lattice is %dx%d", what, m->fnx,
                     m->fny, a->nx, a->ny);
     for (const lbm_bc &b : m->bcs) {
+        if (!uses_bcs) break;
         if (b.kind == LBM_MOVING_WALL) continue;  // acts on the whole lattice
         const bool xe = b.x1 < b.x0, ye = b.y1 < b.y0;
         if ((!xe && (b.x0 < 1 || b.x1 > a->nx)) || (!ye && (b.y0 < 1 || b.y1 > a->ny)))
@@ -298,6 +304,36 @@ lbm::StepParams<double> make_params(const lbm_model_s *m, const lbm_field_s *f, 
     return p;
 }
 
+// largest nodes-per-thread the kernels instantiate (mirrors max_vec in lbm_kernels.h)
+int max_vec(int Q, int storage) {
+    const int eb = storage_bytes(storage);
+    const int cap = eb >= 8 ? 2 : (eb == 4 ? 4 : 8);
+    return Q <= 9 ? cap : (Q <= 17 ? (cap < 4 ? cap : 4) : 2);
+}
+
+// automatic launch shape of the fused step.  Measured on MI355X, D2Q9 TRT
+// 4096^2: 256-thread blocks and non-temporal stores are fastest for every
+// storage; one node per thread for 8- and 4-byte populations, four for the
+// 2-byte storages (one dwordx2 per plane per lane).  lbm_autotune refines
+// this per lattice.
+void auto_launch(const lbm_model_s *m, const lbm_field_s *f, int &block, int &vec, int &nt) {
+    block = m->launch_block > 0 ? m->launch_block : 256;
+    vec = m->launch_vec;
+    nt = m->launch_nt;
+    if (vec <= 0) {
+        vec = 1;
+        if (m->coll != LBM_MRT && storage_bytes(f->storage) == 2) {
+            const int want = max_vec(f->Q, f->storage) < 4 ? max_vec(f->Q, f->storage) : 4;
+            for (int v = want; v > 1; v >>= 1)
+                if (f->nx % v == 0 && f->nx / v >= 64) {
+                    vec = v;
+                    break;
+                }
+        }
+    }
+    if (nt < 0) nt = (m->coll == LBM_MRT) ? 0 : 1;
+}
+
 int run(lbm::Op op, lbm_model m, lbm_field src, lbm_field dst, bool with_bcs) {
     lbm::Launch L{};
     L.op = op;
@@ -306,6 +342,7 @@ int run(lbm::Op op, lbm_model m, lbm_field src, lbm_field dst, bool with_bcs) {
     L.storage = src->storage;
     L.src = src->d;
     L.dst = dst->d;
+    if (op == lbm::Op::Fused) auto_launch(m, src, L.block, L.vec, L.nt);
     const lbm::StepParams<double> p = make_params(m, src, with_bcs);
     const int rc = src->storage == LBM_STORE_F64 ? lbm::dispatch_f64(L, p, g_stream)
                                                  : lbm::dispatch_lp(L, p, g_stream);
@@ -376,9 +413,13 @@ int lbm_field_create(lbm_field *out, int quadrature, int nx, int ny, int storage
     f->Q = q.Q;
     f->storage = storage;
     f->rho0 = rho0;
-    // deviation storages scale g = f - w rho0 by a power of two (exact)
-    f->dscale = (storage == LBM_STORE_F16DEV || storage == LBM_STORE_BF16DEV) ? 1.0 : 1.0;
+    // binary16 storages scale the stored deviation by a power of two (exact)
+    // so that small values stay out of the subnormal range: |f - w rho0| up to
+    // 255 (x 2^8), |f - feq| up to 1 (x 2^16); bfloat16 has FP32's range.
+    f->dscale = storage == LBM_STORE_F16DEV ? 256.0 : (storage == LBM_STORE_F16NEQ ? 65536.0 : 1.0);
     f->bytes = (size_t)nx * (size_t)ny * (size_t)q.Q * (size_t)eb;
+    if (storage == LBM_STORE_F16NEQ || storage == LBM_STORE_BF16NEQ)
+        f->bytes += (size_t)nx * (size_t)ny * 4 * sizeof(float);  // 16-byte moment records
     hipError_t e = hipMalloc(&f->d, f->bytes);
     if (e != hipSuccess) {
         delete f;
@@ -414,6 +455,34 @@ int lbm_field_info(lbm_field f, int *quadrature, int *nx, int *ny, int *Q, int *
     return 0;
 }
 
+// parameters that only depend on the field (quadrature constants, storage)
+static lbm::StepParams<double> field_params(const lbm_field_s *f) {
+    lbm::StepParams<double> p{};
+    HostQuad q;
+    host_quad(f->qid, q);
+    p.nx = f->nx;
+    p.ny = f->ny;
+    p.plane = (long long)f->nx * f->ny;
+    for (int i = 0; i < 37; ++i) p.w[i] = i < q.Q ? q.w[i] : 0.0;
+    const double s = q.s;
+    p.s = s;
+    p.neg_s = -s;
+    p.s2 = s * s;
+    p.three_s = 3.0 * s;
+    p.six_s3 = 6.0 * ((s * s) * s);
+    p.three_s2 = 3.0 * (s * s);
+    p.s4 = pow4(s);
+    p.cs_inv = 1.0 / s;
+    p.rho0 = f->rho0;
+    p.dscale = f->dscale;
+    p.dscale_inv = 1.0 / f->dscale;
+    return p;
+}
+
+static bool is_neq(const lbm_field_s *f) {
+    return f->storage == LBM_STORE_F16NEQ || f->storage == LBM_STORE_BF16NEQ;
+}
+
 static int transfer(lbm_field f, double *host, bool up) {
     if (int rc = check_field(f, up ? "lbm_field_upload" : "lbm_field_download")) return rc;
     if (!host) return fail(-1, "null host array");
@@ -426,16 +495,7 @@ static int transfer(lbm_field f, double *host, bool up) {
     }
     double *tmp = nullptr;
     HIP_TRY(hipMalloc(&tmp, nbytes64));
-    lbm::StepParams<double> p{};
-    HostQuad q;
-    host_quad(f->qid, q);
-    p.nx = f->nx;
-    p.ny = f->ny;
-    p.plane = (long long)f->nx * f->ny;
-    for (int i = 0; i < 37; ++i) p.w[i] = i < q.Q ? q.w[i] : 0.0;
-    p.rho0 = f->rho0;
-    p.dscale = f->dscale;
-    p.dscale_inv = 1.0 / f->dscale;
+    const lbm::StepParams<double> p = field_params(f);
     hipError_t e = hipSuccess;
     if (up) e = hipMemcpyAsync(tmp, host, nbytes64, hipMemcpyHostToDevice, g_stream);
     int rc = 0;
@@ -613,29 +673,29 @@ int lbm_model_set_bcs(lbm_model m, const lbm_bc *bcs, int n) {
 
 // ---- operators --------------------------------------------------------------
 int lbm_collide(lbm_model m, lbm_field f_in, lbm_field f_out) {
-    if (int rc = check_pair(m, f_in, f_out, "lbm_collide", true)) return rc;
+    if (int rc = check_pair(m, f_in, f_out, "lbm_collide", true, false)) return rc;
     return run(lbm::Op::Collide, m, f_in, f_out, false);
 }
 
 int lbm_stream(lbm_model m, lbm_field f, lbm_field f_new) {
-    if (int rc = check_pair(m, f, f_new, "lbm_stream", false)) return rc;
+    if (int rc = check_pair(m, f, f_new, "lbm_stream", false, false)) return rc;
     return run(lbm::Op::StreamBC, m, f, f_new, false);
 }
 
 int lbm_apply_bcs(lbm_model m, lbm_field f_new, lbm_field f_old) {
-    if (int rc = check_pair(m, f_old, f_new, "lbm_apply_bcs", false)) return rc;
+    if (int rc = check_pair(m, f_old, f_new, "lbm_apply_bcs", false, true)) return rc;
     if (m->bcs.empty()) return 0;
     return run(lbm::Op::ApplyBC, m, f_old, f_new, true);
 }
 
 int lbm_stream_bc(lbm_model m, lbm_field f, lbm_field f_new) {
-    if (int rc = check_pair(m, f, f_new, "lbm_stream_bc", false)) return rc;
+    if (int rc = check_pair(m, f, f_new, "lbm_stream_bc", false, true)) return rc;
     return run(lbm::Op::StreamBC, m, f, f_new, true);
 }
 
 static int simulate_impl(lbm_model m, lbm_field S, lbm_field C, int64_t nsteps, double *ms,
                          int64_t *nlaunch) {
-    if (int rc = check_pair(m, S, C, "lbm_simulate", true)) return rc;
+    if (int rc = check_pair(m, S, C, "lbm_simulate", true, true)) return rc;
     if (nsteps < 0) return fail(-1, "negative step count");
     if (ms) *ms = 0.0;
     if (nlaunch) *nlaunch = 0;
@@ -676,6 +736,89 @@ static int simulate_impl(lbm_model m, lbm_field S, lbm_field C, int64_t nsteps, 
     return 0;
 }
 
+int lbm_model_set_launch(lbm_model m, int block_x, int nodes_per_thread, int nontemporal) {
+    if (!m) return fail(-1, "null model");
+    if (block_x != 0 && block_x != 64 && block_x != 128 && block_x != 256)
+        return fail(-1, "block_x must be 0 (automatic), 64, 128 or 256");
+    if (nodes_per_thread != 0 && nodes_per_thread != 1 && nodes_per_thread != 2 &&
+        nodes_per_thread != 4 && nodes_per_thread != 8)
+        return fail(-1, "nodes_per_thread must be 0 (automatic), 1, 2, 4 or 8");
+    m->launch_block = block_x;
+    m->launch_vec = nodes_per_thread;
+    m->launch_nt = nontemporal < 0 ? -1 : (nontemporal ? 1 : 0);
+    return 0;
+}
+
+int lbm_autotune(lbm_model m, lbm_field f_stream, int trial_steps, int *block_x,
+                 int *nodes_per_thread, int *nontemporal, double *us_per_step) {
+    if (!m) return fail(-1, "null model");
+    if (int rc = check_field(f_stream, "lbm_autotune")) return rc;
+    if (f_stream->qid != m->qid) return fail(-1, "lbm_autotune: quadrature mismatch");
+    if (trial_steps < 1) trial_steps = 1;
+    // scratch copies: tuning must not advance the caller's state
+    lbm_field A = nullptr, B = nullptr;
+    int rc = lbm_field_create(&A, f_stream->qid, f_stream->nx, f_stream->ny, f_stream->storage,
+                              f_stream->rho0);
+    if (!rc) rc = lbm_field_create(&B, f_stream->qid, f_stream->nx, f_stream->ny,
+                                   f_stream->storage, f_stream->rho0);
+    if (!rc) rc = lbm_field_copy(A, f_stream);
+    const int save_b = m->launch_block, save_v = m->launch_vec, save_n = m->launch_nt;
+    double best = 1e300;
+    int bb = 0, bv = 1, bn = 0;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (!rc && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
+        rc = fail(-3, "hipEventCreate failed");
+    const int mv = (m->coll == LBM_MRT) ? 1 : max_vec(f_stream->Q, f_stream->storage);
+    const int blocks[3] = {64, 128, 256};
+    for (int vec = 1; !rc && vec <= mv; vec <<= 1) {
+        if (f_stream->nx % vec) continue;
+        for (int nt = 0; !rc && nt <= (m->coll == LBM_MRT ? 0 : 1); ++nt) {
+            for (int bi = 0; !rc && bi < 3; ++bi) {
+                m->launch_block = blocks[bi];
+                m->launch_vec = vec;
+                m->launch_nt = nt;
+                rc = run(lbm::Op::Fused, m, A, B, true);  // warm-up
+                if (rc) break;
+                (void)hipEventRecord(e0, g_stream);
+                lbm_field cur = B, other = A;
+                for (int k = 0; k < trial_steps && !rc; ++k) {
+                    rc = run(lbm::Op::Fused, m, cur, other, true);
+                    std::swap(cur, other);
+                }
+                (void)hipEventRecord(e1, g_stream);
+                if (hipEventSynchronize(e1) != hipSuccess) rc = fail(-3, "autotune sync failed");
+                float ms = 0.0f;
+                (void)hipEventElapsedTime(&ms, e0, e1);
+                const double us = 1e3 * (double)ms / trial_steps;
+                if (!rc && us < best) {
+                    best = us;
+                    bb = blocks[bi];
+                    bv = vec;
+                    bn = nt;
+                }
+            }
+        }
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    lbm_field_destroy(A);
+    lbm_field_destroy(B);
+    if (rc) {
+        m->launch_block = save_b;
+        m->launch_vec = save_v;
+        m->launch_nt = save_n;
+        return rc;
+    }
+    m->launch_block = bb;
+    m->launch_vec = bv;
+    m->launch_nt = bn;
+    if (block_x) *block_x = bb;
+    if (nodes_per_thread) *nodes_per_thread = bv;
+    if (nontemporal) *nontemporal = bn;
+    if (us_per_step) *us_per_step = best;
+    return 0;
+}
+
 int lbm_simulate(lbm_model m, lbm_field f_stream, lbm_field f_collision, int64_t nsteps) {
     return simulate_impl(m, f_stream, f_collision, nsteps, nullptr, nullptr);
 }
@@ -691,20 +834,33 @@ int lbm_simulate_timed(lbm_model m, lbm_field f_stream, lbm_field f_collision, i
 }
 
 // ---- moments ----------------------------------------------------------------
+// NEQ storages: decode into a temporary FP64 field (caller frees *tmp)
+static int as_f64(lbm_field f, const void **data, int *storage, double **tmp) {
+    *tmp = nullptr;
+    *data = f->d;
+    *storage = f->storage;
+    if (!is_neq(f)) return 0;
+    const size_t nb = (size_t)f->nx * f->ny * f->Q * sizeof(double);
+    HIP_TRY(hipMalloc(tmp, nb));
+    const lbm::StepParams<double> p = field_params(f);
+    if (int rc = lbm::dispatch_upload(f->qid, f->storage, *tmp, f->d, p, 0, g_stream)) {
+        (void)hipFree(*tmp);
+        *tmp = nullptr;
+        return fail(-4, "decode kernel failed (%d)", rc);
+    }
+    *data = *tmp;
+    *storage = LBM_STORE_F64;
+    return 0;
+}
+
 int lbm_moments(lbm_field f, double tau, double *rho, double *ux, double *uy, double *sxx,
                 double *sxy, double *syy) {
     if (int rc = check_field(f, "lbm_moments")) return rc;
-    HostQuad q;
-    host_quad(f->qid, q);
-    lbm::StepParams<double> p{};
-    p.nx = f->nx;
-    p.ny = f->ny;
-    p.plane = (long long)f->nx * f->ny;
-    for (int i = 0; i < 37; ++i) p.w[i] = i < q.Q ? q.w[i] : 0.0;
-    p.rho0 = f->rho0;
-    p.dscale = f->dscale;
-    p.dscale_inv = 1.0 / f->dscale;
-    p.cs_inv = 1.0 / q.s;
+    const lbm::StepParams<double> p = field_params(f);
+    const void *fdata;
+    int fstorage;
+    double *ftmp;
+    if (int rc = as_f64(f, &fdata, &fstorage, &ftmp)) return rc;
     double *host[6] = {rho, ux, uy, sxx, sxy, syy};
     double *dev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     const size_t nb = (size_t)p.plane * sizeof(double);
@@ -713,12 +869,13 @@ int lbm_moments(lbm_field f, double tau, double *rho, double *ux, double *uy, do
         if (host[k]) e = hipMalloc(&dev[k], nb);
     int rc = 0;
     if (e == hipSuccess)
-        rc = lbm::dispatch_moments(f->qid, f->storage, f->d, p, tau, dev[0], dev[1], dev[2],
+        rc = lbm::dispatch_moments(f->qid, fstorage, fdata, p, tau, dev[0], dev[1], dev[2],
                                    dev[3], dev[4], dev[5], g_stream);
     for (int k = 0; k < 6 && e == hipSuccess && !rc; ++k)
         if (host[k]) e = hipMemcpyAsync(host[k], dev[k], nb, hipMemcpyDeviceToHost, g_stream);
     if (e == hipSuccess) e = hipStreamSynchronize(g_stream);
     for (int k = 0; k < 6; ++k) free_dev(reinterpret_cast<void *&>(dev[k]));
+    free_dev(reinterpret_cast<void *&>(ftmp));
     if (e != hipSuccess) return fail(-3, "lbm_moments: %s", hipGetErrorString(e));
     if (rc) return fail(-4, "moments kernel failed (%d)", rc);
     return 0;
@@ -727,28 +884,24 @@ int lbm_moments(lbm_field f, double tau, double *rho, double *ux, double *uy, do
 int lbm_reduce(lbm_field f, double *out5) {
     if (int rc = check_field(f, "lbm_reduce")) return rc;
     if (!out5) return fail(-1, "lbm_reduce: null output");
-    HostQuad q;
-    host_quad(f->qid, q);
-    lbm::StepParams<double> p{};
-    p.nx = f->nx;
-    p.ny = f->ny;
-    p.plane = (long long)f->nx * f->ny;
-    for (int i = 0; i < 37; ++i) p.w[i] = i < q.Q ? q.w[i] : 0.0;
-    p.rho0 = f->rho0;
-    p.dscale = f->dscale;
-    p.dscale_inv = 1.0 / f->dscale;
+    const lbm::StepParams<double> p = field_params(f);
+    const void *fdata;
+    int fstorage;
+    double *ftmp;
+    if (int rc = as_f64(f, &fdata, &fstorage, &ftmp)) return rc;
     const int npart = (int)((p.plane + 255) / 256);
     double *part = nullptr, *out = nullptr;
     HIP_TRY(hipMalloc(&part, (size_t)npart * 5 * sizeof(double)));
     hipError_t e = hipMalloc(&out, 8 * sizeof(double));
     int rc = 0;
     if (e == hipSuccess)
-        rc = lbm::dispatch_reduce(f->qid, f->storage, f->d, p, part, npart, out, g_stream);
+        rc = lbm::dispatch_reduce(f->qid, fstorage, fdata, p, part, npart, out, g_stream);
     if (e == hipSuccess && !rc)
         e = hipMemcpyAsync(out5, out, 5 * sizeof(double), hipMemcpyDeviceToHost, g_stream);
     if (e == hipSuccess) e = hipStreamSynchronize(g_stream);
     (void)hipFree(part);
     if (out) (void)hipFree(out);
+    free_dev(reinterpret_cast<void *&>(ftmp));
     if (e != hipSuccess) return fail(-3, "lbm_reduce: %s", hipGetErrorString(e));
     if (rc) return fail(-4, "reduction kernel failed (%d)", rc);
     return 0;

@@ -46,6 +46,41 @@ struct StepParams {
     T cs_inv;                // 1/s
 };
 
+// FP32 copy of the parameters for the FP32-arithmetic storages
+inline StepParams<float> to_float_params(const StepParams<double> &d) {
+    StepParams<float> f{};
+    f.nx = d.nx;
+    f.ny = d.ny;
+    f.plane = d.plane;
+    for (int i = 0; i < 37; ++i) f.w[i] = static_cast<float>(d.w[i]);
+    f.s = static_cast<float>(d.s);
+    f.neg_s = static_cast<float>(d.neg_s);
+    f.s2 = static_cast<float>(d.s2);
+    f.three_s = static_cast<float>(d.three_s);
+    f.six_s3 = static_cast<float>(d.six_s3);
+    f.three_s2 = static_cast<float>(d.three_s2);
+    f.s4 = static_cast<float>(d.s4);
+    f.omo = static_cast<float>(d.omo);
+    f.om = static_cast<float>(d.om);
+    f.oms = static_cast<float>(d.oms);
+    f.oma = static_cast<float>(d.oma);
+    f.tau_force = static_cast<float>(d.tau_force);
+    f.force_kind = d.force_kind;
+    f.Fx = static_cast<float>(d.Fx);
+    f.Fy = static_cast<float>(d.Fy);
+    f.Ffield = d.Ffield;
+    f.nbc = d.nbc;
+    f.bcs = d.bcs;
+    f.mw = d.mw;
+    f.rho0 = static_cast<float>(d.rho0);
+    f.dscale = static_cast<float>(d.dscale);
+    f.dscale_inv = static_cast<float>(d.dscale_inv);
+    f.mrt_n = 0;
+    f.mrt_H = nullptr;
+    f.cs_inv = static_cast<float>(d.cs_inv);
+    return f;
+}
+
 enum class Op : int { Collide = 0, StreamBC = 1, Fused = 2, ApplyBC = 3, Copy = 4, Convert = 5 };
 
 struct Launch {
@@ -56,11 +91,27 @@ struct Launch {
     const void *src;
     void *dst;
     int block;    // threads per block along x (0 = default)
+    int vec = 1;  // fused step: nodes per thread along x (1, 2, 4, 8)
+    int nt = 0;   // fused step: non-temporal stores
 };
 
-// Kernel dispatchers, one per translation unit (FP64 exact / reduced precision).
+// Kernel dispatchers (lbm_dispatch.cpp): FP64 exact / reduced precision.
 int dispatch_f64(const Launch &L, const StepParams<double> &p, hipStream_t s);
 int dispatch_lp(const Launch &L, const StepParams<double> &p, hipStream_t s);
+
+// per (storage, quadrature) launchers, one translation unit per group
+#define LBM_DECLARE_RUN(TAG, T, QID) \
+    int run_##TAG##_q##QID(const Launch &L, const StepParams<T> &p, hipStream_t s);
+#define LBM_DECLARE_ALL_Q(TAG, T)                                                   \
+    LBM_DECLARE_RUN(TAG, T, 4) LBM_DECLARE_RUN(TAG, T, 5) LBM_DECLARE_RUN(TAG, T, 9) \
+    LBM_DECLARE_RUN(TAG, T, 13) LBM_DECLARE_RUN(TAG, T, 17) LBM_DECLARE_RUN(TAG, T, 21) \
+    LBM_DECLARE_RUN(TAG, T, 37)
+LBM_DECLARE_ALL_Q(f64, double)
+LBM_DECLARE_ALL_Q(f32, float)
+LBM_DECLARE_ALL_Q(f16dev, float)
+LBM_DECLARE_ALL_Q(bf16dev, float)
+LBM_DECLARE_ALL_Q(f16neq, float)
+LBM_DECLARE_ALL_Q(bf16neq, float)
 // moments and reductions (FP64 accumulation for every storage)
 int dispatch_moments(int qid, int storage, const void *f, const StepParams<double> &p, double tau,
                      double *rho, double *ux, double *uy, double *sxx, double *sxy, double *syy,

@@ -17,6 +17,7 @@
 #include <hip/hip_fp16.h>
 #include <hip/hip_bf16.h>
 
+#include <type_traits>
 #include <utility>
 
 #include "lbm_internal.h"
@@ -27,56 +28,108 @@ namespace lbm {
 // ---------------------------------------------------------------------------
 // storage traits
 // ---------------------------------------------------------------------------
-struct StF64 {
-    using store_t = double;
-    using T = double;
+// decode(stored, i) -> arithmetic value of population i; encode is its inverse.
+template <class Derived, class S, class TT>
+struct StBase {
+    using store_t = S;
+    using T = TT;
+    static constexpr bool kNeq = false;
+    // kShift: arithmetic runs on shifted populations g_i = f_i - w_i rho0
+    static constexpr bool kShift = !std::is_same<TT, double>::value;
+    __device__ static T load(const S *p, long long idx, int i, const StepParams<T> &q) {
+        return Derived::decode(p[idx], i, q);
+    }
+    __device__ static void store(S *p, long long idx, int i, T v, const StepParams<T> &q) {
+        p[idx] = Derived::encode(v, i, q);
+    }
+};
+
+struct StF64 : StBase<StF64, double, double> {
     static constexpr int id = 0;
-    __device__ static T load(const store_t *p, long long idx, int, const StepParams<T> &) {
-        return p[idx];
-    }
-    __device__ static void store(store_t *p, long long idx, int, T v, const StepParams<T> &) {
-        p[idx] = v;
-    }
+    __device__ static double decode(double s, int, const StepParams<double> &) { return s; }
+    __device__ static double encode(double v, int, const StepParams<double> &) { return v; }
 };
 
-struct StF32 {
-    using store_t = float;
-    using T = float;
+// The FP32-arithmetic storages all work on shifted populations
+// g_i = f_i - w_i rho0: the collision never forms f_i ~ w_i, so FP32 keeps its
+// relative precision on the deviations that carry rho - rho0 and u.
+// FP32 storage: g_i in binary32.
+struct StF32 : StBase<StF32, float, float> {
     static constexpr int id = 1;
-    __device__ static T load(const store_t *p, long long idx, int, const StepParams<T> &) {
-        return p[idx];
-    }
-    __device__ static void store(store_t *p, long long idx, int, T v, const StepParams<T> &) {
-        p[idx] = v;
-    }
+    __device__ static float decode(float s, int, const StepParams<float> &) { return s; }
+    __device__ static float encode(float v, int, const StepParams<float> &) { return v; }
 };
 
-// FP16 deviation storage: g_i = (f_i - w_i rho0) * dscale, stored as IEEE
-// binary16 (round to nearest even); f_i = w_i rho0 + g_i / dscale.
-struct StF16Dev {
-    using store_t = __half;
-    using T = float;
+// FP16 deviation storage: g_i * dscale in IEEE binary16 (round to nearest even).
+struct StF16Dev : StBase<StF16Dev, __half, float> {
     static constexpr int id = 2;
-    __device__ static T load(const store_t *p, long long idx, int i, const StepParams<T> &q) {
-        return q.w[i] * q.rho0 + __half2float(p[idx]) * q.dscale_inv;
+    __device__ static float decode(__half s, int, const StepParams<float> &q) {
+        return __half2float(s) * q.dscale_inv;
     }
-    __device__ static void store(store_t *p, long long idx, int i, T v, const StepParams<T> &q) {
-        p[idx] = __float2half_rn((v - q.w[i] * q.rho0) * q.dscale);
+    __device__ static __half encode(float v, int, const StepParams<float> &q) {
+        return __float2half_rn(v * q.dscale);
     }
 };
 
 // BF16 deviation storage, same scheme with bfloat16 (8-bit significand).
-struct StBF16Dev {
-    using store_t = __hip_bfloat16;
-    using T = float;
+struct StBF16Dev : StBase<StBF16Dev, __hip_bfloat16, float> {
     static constexpr int id = 3;
-    __device__ static T load(const store_t *p, long long idx, int i, const StepParams<T> &q) {
-        return q.w[i] * q.rho0 + __bfloat162float(p[idx]) * q.dscale_inv;
+    __device__ static float decode(__hip_bfloat16 s, int, const StepParams<float> &q) {
+        return __bfloat162float(s) * q.dscale_inv;
     }
-    __device__ static void store(store_t *p, long long idx, int i, T v, const StepParams<T> &q) {
-        p[idx] = __float2bfloat16((v - q.w[i] * q.rho0) * q.dscale);
+    __device__ static __hip_bfloat16 encode(float v, int, const StepParams<float> &q) {
+        return __float2bfloat16(v * q.dscale);
     }
 };
+
+// Non-equilibrium storage: per node one 16-byte record of FP32 moments
+// (rho - rho0, ux, uy, unused) of the stored population vector and, per
+// population, the 16-bit non-equilibrium part (f_i - feq_i(rho, u)) * dscale;
+// the arithmetic value is g_i = f_i - w_i rho0 = (feq_i - w_i rho0) + (f_i - feq_i).
+// The field holds the node records first (one dwordx4 per neighbour read),
+// then Q 16-bit planes.
+struct alignas(16) Mom4 {
+    float drho, ux, uy, pad;
+};
+
+template <class S, int ID>
+struct StNeq {
+    using store_t = S;
+    using T = float;
+    static constexpr int id = ID;
+    static constexpr bool kNeq = true;
+    static constexpr bool kShift = true;
+    __device__ static float to_f(S v, const StepParams<float> &q) {
+        if constexpr (ID == 4) return __half2float(v) * q.dscale_inv;
+        else return __bfloat162float(v) * q.dscale_inv;
+    }
+    __device__ static S from_f(float v, const StepParams<float> &q) {
+        if constexpr (ID == 4) return __float2half_rn(v * q.dscale);
+        else return __float2bfloat16(v * q.dscale);
+    }
+    __device__ static const Mom4 *mom(const S *base) { return reinterpret_cast<const Mom4 *>(base); }
+    __device__ static Mom4 *mom(S *base) { return reinterpret_cast<Mom4 *>(base); }
+    // population plane 0 starts after the 16-byte records = 8 * plane 16-bit words
+    __device__ static const S *pops(const S *base, long long plane) { return base + 8 * plane; }
+    __device__ static S *pops(S *base, long long plane) { return base + 8 * plane; }
+};
+using StF16Neq = StNeq<__half, 4>;
+using StBF16Neq = StNeq<__hip_bfloat16, 5>;
+
+// VEC consecutive stored values of one plane (one thread's x-run)
+template <class S, int VEC>
+struct alignas(sizeof(S)) Pack {
+    S v[VEC];
+};
+
+// raw bit container of a Pack for non-temporal stores
+template <int BYTES>
+struct RawOf;
+template <> struct RawOf<2> { using type = unsigned short; };
+template <> struct RawOf<4> { using type = unsigned int; };
+template <> struct RawOf<8> { using type = unsigned int __attribute__((ext_vector_type(2))); };
+template <> struct RawOf<16> { using type = unsigned int __attribute__((ext_vector_type(4))); };
+template <> struct RawOf<32> { using type = unsigned int __attribute__((ext_vector_type(8))); };
 
 // ---------------------------------------------------------------------------
 // small helpers
@@ -149,6 +202,90 @@ __device__ __forceinline__ T feq_i(T rho, T ux, T uy, T usq, const StepParams<T>
     }
 }
 
+template <int QID, class T, int... Is>
+__device__ __forceinline__ void feq_all_seq(T rho, T ux, T uy, const StepParams<T> &p,
+                                            T (&feq)[QT<QID>::Q],
+                                            std::integer_sequence<int, Is...>) {
+    const T usq = ux * ux + uy * uy;
+    ((feq[Is] = feq_i<QID, Is, T>(rho, ux, uy, usq, p)), ...);
+}
+
+template <int QID, class T>
+__device__ __forceinline__ void feq_all(T rho, T ux, T uy, const StepParams<T> &p,
+                                        T (&feq)[QT<QID>::Q]) {
+    feq_all_seq<QID, T>(rho, ux, uy, p, feq, std::make_integer_sequence<int, QT<QID>::Q>{});
+}
+
+// population I of node n of field F, in arithmetic precision
+template <int QID, int I, class ST>
+__device__ __forceinline__ typename ST::T ld_pop(const typename ST::store_t *__restrict__ F,
+                                                 const StepParams<typename ST::T> &p,
+                                                 long long n) {
+    if constexpr (ST::kNeq) {
+        const Mom4 m = ST::mom(F)[n];
+        const float usq = m.ux * m.ux + m.uy * m.uy;
+        return feq_dev_i<QID, I, float>(m.drho, p.rho0 + m.drho, m.ux, m.uy, usq, p) +
+               ST::to_f(ST::pops(F, p.plane)[p.plane * I + n], p);
+    } else {
+        return ST::load(F + p.plane * I, n, I, p);
+    }
+}
+
+// feq_i - w_i rho0 = w_i (drho + rho (a1 + a2/2 + a3/6 + a4/24)), the shifted
+// equilibrium of the FP32 paths
+template <int QID, int I, class T>
+__device__ __forceinline__ T feq_dev_i(T drho, T rho, T ux, T uy, T usq, const StepParams<T> &p) {
+    constexpr int FEQ = QT<QID>::FEQ;
+    const T ud = udot<QID, I>(ux, uy);
+    const T a1 = p.s * ud;
+    T poly = a1;
+    if constexpr (FEQ >= 2) {
+        const T ud2 = ud * ud;
+        poly = poly + T(0.5) * (p.s2 * ud2 + p.neg_s * usq);
+        if constexpr (FEQ >= 3) poly = poly + T(1.0 / 6.0) * ((p.s * ud) * (p.s2 * ud2 - p.three_s * usq));
+        if constexpr (FEQ >= 4)
+            poly = poly + T(1.0 / 24.0) * ((p.s4 * (ud2 * ud2) - (p.six_s3 * usq) * ud2) +
+                                           p.three_s2 * (usq * usq));
+    }
+    return p.w[I] * (drho + rho * poly);
+}
+
+template <int QID, class T, int... Is>
+__device__ __forceinline__ void feq_dev_all_seq(T drho, T ux, T uy, const StepParams<T> &p,
+                                                T (&geq)[QT<QID>::Q],
+                                                std::integer_sequence<int, Is...>) {
+    const T usq = ux * ux + uy * uy;
+    const T rho = p.rho0 + drho;
+    ((geq[Is] = feq_dev_i<QID, Is, T>(drho, rho, ux, uy, usq, p)), ...);
+}
+
+template <int QID, class T>
+__device__ __forceinline__ void feq_dev_all(T drho, T ux, T uy, const StepParams<T> &p,
+                                            T (&geq)[QT<QID>::Q]) {
+    feq_dev_all_seq<QID, T>(drho, ux, uy, p, geq, std::make_integer_sequence<int, QT<QID>::Q>{});
+}
+
+// shifted moments: drho = sum g, rho u = sum c g (sum_i c_i w_i = 0)
+template <int QID, class T, int... Is>
+__device__ __forceinline__ void moments_dev_seq(const T (&g)[QT<QID>::Q], const StepParams<T> &p,
+                                                T &drho, T &ux, T &uy,
+                                                std::integer_sequence<int, Is...>) {
+    T r = T(0), mx = T(0), my = T(0);
+    ((r = r + g[Is]), ...);
+    ((mx = (QT<QID>::cx(Is) != 0 ? mx + cmul<QT<QID>::cx(Is)>(g[Is]) : mx)), ...);
+    ((my = (QT<QID>::cy(Is) != 0 ? my + cmul<QT<QID>::cy(Is)>(g[Is]) : my)), ...);
+    const T rho = p.rho0 + r;
+    drho = r;
+    ux = mx / rho;
+    uy = my / rho;
+}
+
+template <int QID, class T>
+__device__ __forceinline__ void node_moments_dev(const T (&g)[QT<QID>::Q], const StepParams<T> &p,
+                                                 T &drho, T &ux, T &uy) {
+    moments_dev_seq<QID, T>(g, p, drho, ux, uy, std::make_integer_sequence<int, QT<QID>::Q>{});
+}
+
 // ---------------------------------------------------------------------------
 // per-node moments: density = sum(f) (moments.jl:3), velocity! (moments.jl:5-19)
 // ---------------------------------------------------------------------------
@@ -185,12 +322,17 @@ __device__ __forceinline__ void node_force(const StepParams<T> &p, int x, int y,
 // ---------------------------------------------------------------------------
 // collision of one node, in place (f -> post-collision f)
 // ---------------------------------------------------------------------------
-template <int QID, int COLL, class T, int... Is>
+template <int QID, int COLL, bool SHIFT, class T, int... Is>
 __device__ __forceinline__ void collide_seq(T (&f)[QT<QID>::Q], const StepParams<T> &p, int x,
                                             int y, std::integer_sequence<int, Is...>) {
     constexpr int Q = QT<QID>::Q;
-    T rho, ux, uy;
-    node_moments<QID, T>(f, rho, ux, uy);
+    T rho, ux, uy, drho = T(0);
+    if constexpr (SHIFT) {
+        node_moments_dev<QID, T>(f, p, drho, ux, uy);
+        rho = p.rho0 + drho;
+    } else {
+        node_moments<QID, T>(f, rho, ux, uy);
+    }
     if (p.force_kind != 0) {
         T Fx, Fy;
         node_force(p, x, y, Fx, Fy);
@@ -199,7 +341,8 @@ __device__ __forceinline__ void collide_seq(T (&f)[QT<QID>::Q], const StepParams
     }
     const T usq = ux * ux + uy * uy;
     T feq[Q];
-    ((feq[Is] = feq_i<QID, Is, T>(rho, ux, uy, usq, p)), ...);
+    if constexpr (SHIFT) ((feq[Is] = feq_dev_i<QID, Is, T>(drho, rho, ux, uy, usq, p)), ...);
+    else ((feq[Is] = feq_i<QID, Is, T>(rho, ux, uy, usq, p)), ...);
     if constexpr (COLL == 0) {
         // srt.jl:76-78
         ((f[Is] = p.omo * f[Is] + p.om * feq[Is]), ...);
@@ -220,10 +363,10 @@ __device__ __forceinline__ void collide_seq(T (&f)[QT<QID>::Q], const StepParams
     }
 }
 
-template <int QID, int COLL, class T>
+template <int QID, int COLL, bool SHIFT, class T>
 __device__ __forceinline__ void collide_node(T (&f)[QT<QID>::Q], const StepParams<T> &p, int x,
                                              int y) {
-    collide_seq<QID, COLL, T>(f, p, x, y, std::make_integer_sequence<int, QT<QID>::Q>{});
+    collide_seq<QID, COLL, SHIFT, T>(f, p, x, y, std::make_integer_sequence<int, QT<QID>::Q>{});
 }
 
 // ---------------------------------------------------------------------------
@@ -313,7 +456,7 @@ __device__ __forceinline__ void apply_bc_seq(const typename ST::store_t *__restr
             default: hit = (x - cx) < 0; break;
             }
             if (hit) {
-                T v = ST::load(C + p.plane * j, node, j, p);
+                T v = ld_pop<QID, j, ST>(C, p, node);
                 if (bc.kind == 1) v = v + static_cast<T>(p.mw[b * Q + i]);
                 f[i] = v;
             }
@@ -334,15 +477,15 @@ __device__ __forceinline__ void pull_seq(const typename ST::store_t *__restrict_
     const bool edge = (x < M) | (x >= nx - M) | (y < M) | (y >= ny - M);
     if (!edge) {
         const long long node = (long long)x + (long long)nx * y;
-        ((f[Is] = ST::load(C + p.plane * Is,
-                           node - QT<QID>::cx(Is) - (long long)nx * QT<QID>::cy(Is), Is, p)),
+        ((f[Is] = ld_pop<QID, Is, ST>(
+              C, p, node - QT<QID>::cx(Is) - (long long)nx * QT<QID>::cy(Is))),
          ...);
     } else {
         auto one = [&](auto Ic) {
             constexpr int i = decltype(Ic)::value;
             const int sx = wrap(x - QT<QID>::cx(i), nx);
             const int sy = wrap(y - QT<QID>::cy(i), ny);
-            f[i] = ST::load(C + p.plane * i, (long long)sx + (long long)nx * sy, i, p);
+            f[i] = ld_pop<QID, i, ST>(C, p, (long long)sx + (long long)nx * sy);
         };
         (one(std::integral_constant<int, Is>{}), ...);
         if (p.nbc > 0) apply_bc_seq<QID, ST>(C, p, x, y, f, seq);
@@ -354,7 +497,20 @@ __device__ __forceinline__ void load_local(const typename ST::store_t *__restric
                                            const StepParams<typename ST::T> &p, long long node,
                                            typename ST::T (&f)[QT<QID>::Q],
                                            std::integer_sequence<int, Is...>) {
-    ((f[Is] = ST::load(F + p.plane * Is, node, Is, p)), ...);
+    ((f[Is] = ld_pop<QID, Is, ST>(F, p, node)), ...);
+}
+
+// moments of a population vector and its non-equilibrium parts (FP32)
+// moments (rho - rho0, u) of a shifted population vector and its
+// non-equilibrium parts f_i - feq_i
+template <int QID, class T>
+__device__ __forceinline__ void neq_split(const T (&f)[QT<QID>::Q], const StepParams<T> &p,
+                                          T &drho, T &ux, T &uy, T (&g)[QT<QID>::Q]) {
+    node_moments_dev<QID, T>(f, p, drho, ux, uy);
+    T geq[QT<QID>::Q];
+    feq_dev_all<QID, T>(drho, ux, uy, p, geq);
+#pragma unroll
+    for (int i = 0; i < QT<QID>::Q; ++i) g[i] = f[i] - geq[i];
 }
 
 template <int QID, class ST, int... Is>
@@ -362,14 +518,27 @@ __device__ __forceinline__ void store_local(typename ST::store_t *__restrict__ F
                                             const StepParams<typename ST::T> &p, long long node,
                                             const typename ST::T (&f)[QT<QID>::Q],
                                             std::integer_sequence<int, Is...>) {
-    ((ST::store(F + p.plane * Is, node, Is, f[Is], p)), ...);
+    if constexpr (ST::kNeq) {
+        Mom4 m;
+        float g[QT<QID>::Q];
+        neq_split<QID, float>(f, p, m.drho, m.ux, m.uy, g);
+        m.pad = 0.0f;
+        ST::mom(F)[node] = m;
+        auto *P = ST::pops(F, p.plane);
+        ((P[p.plane * Is + node] = ST::from_f(g[Is], p)), ...);
+    } else {
+        ((ST::store(F + p.plane * Is, node, Is, f[Is], p)), ...);
+    }
 }
 
+// FP64 runs on the populations themselves (the reference's arithmetic); the
+// FP32 storages on shifted populations
 template <int QID, int COLL, class T>
 __device__ __forceinline__ void collide_any(T (&f)[QT<QID>::Q], const StepParams<T> &p, int x,
                                             int y) {
+    constexpr bool SHIFT = !std::is_same<T, double>::value;
     if constexpr (COLL == 2) collide_mrt_node<QID, T>(f, p, x, y);
-    else collide_node<QID, COLL, T>(f, p, x, y);
+    else collide_node<QID, COLL, SHIFT, T>(f, p, x, y);
 }
 
 // ---------------------------------------------------------------------------
@@ -419,6 +588,207 @@ k_fused(const typename ST::store_t *__restrict__ C, typename ST::store_t *__rest
     store_local<QID, ST>(Cn, p, (long long)x + (long long)p.nx * y, f, LBM_SEQ);
 }
 
+// ---------------------------------------------------------------------------
+// fused step, VEC consecutive nodes per thread.  In the interior every plane
+// is read as one VEC-wide (unaligned for shifted planes) vector load and
+// written as one aligned vector store; requires nx % VEC == 0 (host-checked).
+// NT: non-temporal stores (the written lattice is not re-read this step).
+// ---------------------------------------------------------------------------
+template <int QID, class ST, int VEC, int... Is>
+__device__ __forceinline__ void pull_interior_vec(const typename ST::store_t *__restrict__ C,
+                                                  const StepParams<typename ST::T> &p,
+                                                  long long node0,
+                                                  typename ST::T (&f)[VEC][QT<QID>::Q],
+                                                  std::integer_sequence<int, Is...>) {
+    using S = typename ST::store_t;
+    auto one = [&](auto Ic) {
+        constexpr int i = decltype(Ic)::value;
+        const long long src = node0 - QT<QID>::cx(i) - (long long)p.nx * QT<QID>::cy(i);
+        Pack<S, VEC> pk;
+        if constexpr (ST::kNeq) {
+            __builtin_memcpy(&pk, ST::pops(C, p.plane) + p.plane * i + src, sizeof(pk));
+            const Mom4 *m = ST::mom(C) + src;
+#pragma unroll
+            for (int v = 0; v < VEC; ++v) {
+                const Mom4 mv = m[v];
+                const float usq = mv.ux * mv.ux + mv.uy * mv.uy;
+                f[v][i] = feq_dev_i<QID, i, float>(mv.drho, p.rho0 + mv.drho, mv.ux, mv.uy, usq,
+                                                   p) +
+                          ST::to_f(pk.v[v], p);
+            }
+        } else {
+            __builtin_memcpy(&pk, C + p.plane * i + src, sizeof(pk));
+#pragma unroll
+            for (int v = 0; v < VEC; ++v) f[v][i] = ST::decode(pk.v[v], i, p);
+        }
+    };
+    (one(std::integral_constant<int, Is>{}), ...);
+}
+
+template <bool NT, class S, int VEC>
+__device__ __forceinline__ void put_pack(S *dst, const Pack<S, VEC> &pk) {
+    if constexpr (NT) {
+        using R = typename RawOf<sizeof(pk)>::type;
+        R r;
+        __builtin_memcpy(&r, &pk, sizeof(pk));
+        __builtin_nontemporal_store(r, reinterpret_cast<R *>(dst));
+    } else {
+        __builtin_memcpy(dst, &pk, sizeof(pk));
+    }
+}
+
+template <int QID, class ST, int VEC, bool NT, int... Is>
+__device__ __forceinline__ void store_vec(typename ST::store_t *__restrict__ F,
+                                          const StepParams<typename ST::T> &p, long long node0,
+                                          const typename ST::T (&f)[VEC][QT<QID>::Q],
+                                          std::integer_sequence<int, Is...>) {
+    using S = typename ST::store_t;
+    if constexpr (ST::kNeq) {
+        constexpr int Q = QT<QID>::Q;
+        float g[VEC][Q];
+        Mom4 *m = ST::mom(F) + node0;
+#pragma unroll
+        for (int v = 0; v < VEC; ++v) {
+            Mom4 mv;
+            neq_split<QID, float>(f[v], p, mv.drho, mv.ux, mv.uy, g[v]);
+            mv.pad = 0.0f;
+            Pack<Mom4, 1> pm;
+            pm.v[0] = mv;
+            put_pack<NT>(m + v, pm);
+        }
+        auto one = [&](auto Ic) {
+            constexpr int i = decltype(Ic)::value;
+            Pack<S, VEC> pk;
+#pragma unroll
+            for (int v = 0; v < VEC; ++v) pk.v[v] = ST::from_f(g[v][i], p);
+            put_pack<NT>(ST::pops(F, p.plane) + p.plane * i + node0, pk);
+        };
+        (one(std::integral_constant<int, Is>{}), ...);
+    } else {
+        auto one = [&](auto Ic) {
+            constexpr int i = decltype(Ic)::value;
+            Pack<S, VEC> pk;
+#pragma unroll
+            for (int v = 0; v < VEC; ++v) pk.v[v] = ST::encode(f[v][i], i, p);
+            put_pack<NT>(F + p.plane * i + node0, pk);
+        };
+        (one(std::integral_constant<int, Is>{}), ...);
+    }
+}
+
+template <int QID, int COLL, class ST, int VEC, bool NT>
+__global__ void __launch_bounds__(256)
+k_fused_v(const typename ST::store_t *__restrict__ C, typename ST::store_t *__restrict__ Cn,
+          const StepParams<typename ST::T> p) {
+    constexpr int M = QT<QID>::MAXC;
+    const int x0 = (blockIdx.x * blockDim.x + threadIdx.x) * VEC;
+    const int y = blockIdx.y;
+    if (x0 >= p.nx) return;
+    typename ST::T f[VEC][QT<QID>::Q];
+    const long long node0 = (long long)x0 + (long long)p.nx * y;
+    const bool interior = (x0 >= M) && (x0 + VEC <= p.nx - M) && (y >= M) && (y < p.ny - M);
+    if (interior) {
+        pull_interior_vec<QID, ST, VEC>(C, p, node0, f, LBM_SEQ);
+    } else {
+#pragma unroll
+        for (int v = 0; v < VEC; ++v) pull_seq<QID, ST>(C, p, x0 + v, y, f[v], LBM_SEQ);
+    }
+#pragma unroll
+    for (int v = 0; v < VEC; ++v) collide_any<QID, COLL>(f[v], p, x0 + v, y);
+    store_vec<QID, ST, VEC, NT>(Cn, p, node0, f, LBM_SEQ);
+}
+
+// ---------------------------------------------------------------------------
+// fused step for the non-equilibrium storages with LDS staging: the block
+// (one row, blockDim.x * VEC nodes) first copies the moment records of rows
+// y - M .. y + M over its x-range plus an M-node halo into LDS (periodic
+// wrap applied there), so every neighbour's moments cost one LDS read; the
+// 16-bit non-equilibrium planes are read once each from HBM as before.
+// ---------------------------------------------------------------------------
+template <int QID, int VEC>
+constexpr int neq_tile_cols() { return 256 * VEC + 2 * QT<QID>::MAXC; }
+
+template <int QID, int VEC>
+constexpr bool neq_lds_fits() {
+    return (2 * QT<QID>::MAXC + 1) * neq_tile_cols<QID, VEC>() * (int)sizeof(Mom4) <= 64 * 1024;
+}
+
+template <int QID, class ST, int VEC, int... Is>
+__device__ __forceinline__ void pull_neq_lds(const typename ST::store_t *__restrict__ C,
+                                             const StepParams<float> &p,
+                                             const Mom4 (*tile)[neq_tile_cols<QID, VEC>()],
+                                             int xb, int x0, int y, bool interior,
+                                             float (&f)[VEC][QT<QID>::Q],
+                                             std::integer_sequence<int, Is...>) {
+    using S = typename ST::store_t;
+    constexpr int M = QT<QID>::MAXC;
+    const int nx = p.nx, ny = p.ny;
+    const long long node0 = (long long)x0 + (long long)nx * y;
+    auto one = [&](auto Ic) {
+        constexpr int i = decltype(Ic)::value;
+        constexpr int cx = QT<QID>::cx(i), cy = QT<QID>::cy(i);
+        Pack<S, VEC> pk;
+        if (interior) {
+            __builtin_memcpy(&pk, ST::pops(C, p.plane) + p.plane * i + node0 - cx - (long long)nx * cy,
+                             sizeof(pk));
+        } else {
+#pragma unroll
+            for (int v = 0; v < VEC; ++v) {
+                const int sx = wrap(x0 + v - cx, nx), sy = wrap(y - cy, ny);
+                pk.v[v] = ST::pops(C, p.plane)[p.plane * i + (long long)sx + (long long)nx * sy];
+            }
+        }
+#pragma unroll
+        for (int v = 0; v < VEC; ++v) {
+            const Mom4 mv = tile[M - cy][x0 + v - xb + M - cx];
+            const float usq = mv.ux * mv.ux + mv.uy * mv.uy;
+            f[v][i] = feq_dev_i<QID, i, float>(mv.drho, p.rho0 + mv.drho, mv.ux, mv.uy, usq, p) +
+                      ST::to_f(pk.v[v], p);
+        }
+    };
+    (one(std::integral_constant<int, Is>{}), ...);
+}
+
+template <int QID, int COLL, class ST, int VEC, bool NT>
+__global__ void __launch_bounds__(256)
+k_fused_neq(const typename ST::store_t *__restrict__ C, typename ST::store_t *__restrict__ Cn,
+            const StepParams<float> p) {
+    constexpr int M = QT<QID>::MAXC;
+    constexpr int COLS = neq_tile_cols<QID, VEC>();
+    __shared__ Mom4 tile[2 * M + 1][COLS];
+    const int nx = p.nx, ny = p.ny;
+    const int y = blockIdx.y;
+    const int xb = blockIdx.x * blockDim.x * VEC;
+    const int span = blockDim.x * VEC + 2 * M;  // <= COLS
+    const Mom4 *mom = ST::mom(C);
+    for (int r = 0; r < 2 * M + 1; ++r) {
+        const long long row = (long long)nx * wrap(y - M + r, ny);
+        for (int k = threadIdx.x; k < span; k += blockDim.x)
+            tile[r][k] = mom[row + wrap(xb - M + k, nx)];
+    }
+    __syncthreads();
+    const int x0 = xb + threadIdx.x * VEC;
+    if (x0 >= nx) return;
+    const bool interior = (x0 >= M) && (x0 + VEC <= nx - M) && (y >= M) && (y < ny - M);
+    float f[VEC][QT<QID>::Q];
+    pull_neq_lds<QID, ST, VEC>(C, p, tile, xb, x0, y, interior, f, LBM_SEQ);
+    if (!interior && p.nbc > 0) {
+#pragma unroll
+        for (int v = 0; v < VEC; ++v) apply_bc_seq<QID, ST>(C, p, x0 + v, y, f[v], LBM_SEQ);
+    }
+#pragma unroll
+    for (int v = 0; v < VEC; ++v) collide_any<QID, COLL>(f[v], p, x0 + v, y);
+    store_vec<QID, ST, VEC, NT>(Cn, p, (long long)x0 + (long long)nx * y, f, LBM_SEQ);
+}
+
+// largest nodes-per-thread worth instantiating (keeps VEC * Q registers bounded)
+template <int QID, class ST>
+constexpr int max_vec() {
+    constexpr int Q = QT<QID>::Q;
+    constexpr int cap = sizeof(typename ST::store_t) >= 8 ? 2 : (sizeof(typename ST::store_t) == 4 ? 4 : 8);
+    return (Q <= 9) ? cap : (Q <= 17 ? (cap < 4 ? cap : 4) : 2);
+}
+
 // apply!(bcs, q, f_new, f_old) in place on f_new (only nodes within MAXC of an edge)
 template <int QID, class ST, int... Is>
 __device__ __forceinline__ void apply_only(const typename ST::store_t *__restrict__ fold,
@@ -447,7 +817,7 @@ __device__ __forceinline__ void apply_only(const typename ST::store_t *__restric
             default: hit = (x - cx) < 0; break;
             }
             if (hit) {
-                T v = ST::load(fold + p.plane * j, node, j, p);
+                T v = ld_pop<QID, j, ST>(fold, p, node);
                 if (bc.kind == 1) v = v + static_cast<T>(p.mw[b * Q + i]);
                 f[i] = v;
                 touched[i] = true;
@@ -455,11 +825,22 @@ __device__ __forceinline__ void apply_only(const typename ST::store_t *__restric
         };
         (one(std::integral_constant<int, Is>{}), ...);
     }
-    auto st = [&](auto Ic) {
-        constexpr int i = decltype(Ic)::value;
-        if (touched[i]) ST::store(fnew + p.plane * i, node, i, f[i], p);
-    };
-    (st(std::integral_constant<int, Is>{}), ...);
+    if constexpr (ST::kNeq) {
+        // a node's 16-bit parts are relative to its own moments: re-encode the node
+        bool any = false;
+        ((any = any || touched[Is]), ...);
+        if (!any) return;
+        T g[Q];
+        load_local<QID, ST>(fnew, p, node, g, seq);
+        ((g[Is] = touched[Is] ? f[Is] : g[Is]), ...);
+        store_local<QID, ST>(fnew, p, node, g, seq);
+    } else {
+        auto st = [&](auto Ic) {
+            constexpr int i = decltype(Ic)::value;
+            if (touched[i]) ST::store(fnew + p.plane * i, node, i, f[i], p);
+        };
+        (st(std::integral_constant<int, Is>{}), ...);
+    }
 }
 
 template <int QID, class ST>
@@ -495,6 +876,45 @@ inline int default_block(int nx) {
     return 64;
 }
 
+template <int QID, int COLL, class ST, int VEC>
+int launch_fused_vn(const Launch &L, const StepParams<typename ST::T> &p, hipStream_t s) {
+    using S = typename ST::store_t;
+    const int bx = L.block > 0 ? L.block : default_block(p.nx / VEC);
+    const dim3 grid((p.nx / VEC + bx - 1) / bx, p.ny, 1), block(bx, 1, 1);
+    const S *src = static_cast<const S *>(L.src);
+    S *dst = static_cast<S *>(L.dst);
+    if constexpr (ST::kNeq) {
+        if constexpr (neq_lds_fits<QID, VEC>()) {
+            if (L.nt) k_fused_neq<QID, COLL, ST, VEC, true><<<grid, block, 0, s>>>(src, dst, p);
+            else k_fused_neq<QID, COLL, ST, VEC, false><<<grid, block, 0, s>>>(src, dst, p);
+            return hipGetLastError() == hipSuccess ? 0 : -4;
+        }
+    }
+    if (L.nt) k_fused_v<QID, COLL, ST, VEC, true><<<grid, block, 0, s>>>(src, dst, p);
+    else k_fused_v<QID, COLL, ST, VEC, false><<<grid, block, 0, s>>>(src, dst, p);
+    return hipGetLastError() == hipSuccess ? 0 : -4;
+}
+
+template <int QID, int COLL, class ST>
+int launch_fused_c(const Launch &L, const StepParams<typename ST::T> &p, hipStream_t s) {
+    constexpr int MV = max_vec<QID, ST>();
+    const int vec = L.vec < 1 ? 1 : L.vec;
+    if (vec > MV || p.nx % vec != 0) return -5;
+    switch (vec) {
+    case 1: return launch_fused_vn<QID, COLL, ST, 1>(L, p, s);
+    case 2: if constexpr (MV >= 2) return launch_fused_vn<QID, COLL, ST, 2>(L, p, s); break;
+    case 4: if constexpr (MV >= 4) return launch_fused_vn<QID, COLL, ST, 4>(L, p, s); break;
+    case 8: if constexpr (MV >= 8) return launch_fused_vn<QID, COLL, ST, 8>(L, p, s); break;
+    default: break;
+    }
+    return -5;
+}
+
+template <int QID, class ST>
+int launch_fused_v(const Launch &L, const StepParams<typename ST::T> &p, hipStream_t s) {
+    return L.coll == 0 ? launch_fused_c<QID, 0, ST>(L, p, s) : launch_fused_c<QID, 1, ST>(L, p, s);
+}
+
 template <int QID, class ST>
 int launch_op(const Launch &L, const StepParams<typename ST::T> &p, hipStream_t s) {
     using S = typename ST::store_t;
@@ -510,10 +930,14 @@ int launch_op(const Launch &L, const StepParams<typename ST::T> &p, hipStream_t 
         else return -2;
         break;
     case Op::Fused:
+        if (L.coll == 2) {
+            if constexpr (ST::id == 0) k_fused<QID, 2, ST><<<grid, block, 0, s>>>(src, dst, p);
+            else return -2;
+            break;
+        }
+        if (L.vec > 1 || L.nt || ST::kNeq) return launch_fused_v<QID, ST>(L, p, s);
         if (L.coll == 0) k_fused<QID, 0, ST><<<grid, block, 0, s>>>(src, dst, p);
-        else if (L.coll == 1) k_fused<QID, 1, ST><<<grid, block, 0, s>>>(src, dst, p);
-        else if constexpr (ST::id == 0) k_fused<QID, 2, ST><<<grid, block, 0, s>>>(src, dst, p);
-        else return -2;
+        else k_fused<QID, 1, ST><<<grid, block, 0, s>>>(src, dst, p);
         break;
     case Op::StreamBC:
         k_stream_bc<QID, ST><<<grid, block, 0, s>>>(src, dst, p);
@@ -527,18 +951,12 @@ int launch_op(const Launch &L, const StepParams<typename ST::T> &p, hipStream_t 
     return hipGetLastError() == hipSuccess ? 0 : -4;
 }
 
-template <class ST>
-int dispatch_q(const Launch &L, const StepParams<typename ST::T> &p, hipStream_t s) {
-    switch (L.qid) {
-    case 4: return launch_op<4, ST>(L, p, s);
-    case 5: return launch_op<5, ST>(L, p, s);
-    case 9: return launch_op<9, ST>(L, p, s);
-    case 13: return launch_op<13, ST>(L, p, s);
-    case 17: return launch_op<17, ST>(L, p, s);
-    case 21: return launch_op<21, ST>(L, p, s);
-    case 37: return launch_op<37, ST>(L, p, s);
-    default: return -1;
+// One translation unit per (storage, quadrature group) instantiates these
+// entry points; lbm_dispatch.cpp routes a launch to them.
+#define LBM_DEFINE_RUN(TAG, ST, QID)                                                        \
+    int run_##TAG##_q##QID(const Launch &L, const StepParams<typename ST::T> &p,           \
+                           hipStream_t s) {                                                 \
+        return launch_op<QID, ST>(L, p, s);                                                 \
     }
-}
 
 }  // namespace lbm

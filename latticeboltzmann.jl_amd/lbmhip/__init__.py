@@ -13,7 +13,7 @@ from .collision_models import (MRT, SRT, TRT, TRT_Λ, collision_model_from_visco
 from .initial_conditions import cell_centres, equilibrium_populations  # noqa: F401
 from .lattice import HipLattice  # noqa: F401
 from .model import (LatticeBoltzmannModel, ProcessingMethod, apply_boundary_conditions_,  # noqa: F401
-                    collide_model_, next_, simulate, simulate_timed, stream_model_)
+                    autotune, collide_model_, next_, simulate, simulate_timed, stream_model_)
 from .moments import conserved_sums, density, moments, velocity  # noqa: F401
 from .operators import apply_, collide_, stream_, stream_bc_, synchronize  # noqa: F401
 from .quadratures import (D2Q4, D2Q5, D2Q9, D2Q13, D2Q17, D2Q21, D2Q37, Quadrature,  # noqa: F401

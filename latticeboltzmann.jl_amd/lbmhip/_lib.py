@@ -12,8 +12,9 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("LBM_HIP_LIB",
                           os.path.join(os.path.dirname(_HERE), "lib", "liblbm_hip.so"))
 
-STORE_F64, STORE_F32, STORE_F16DEV, STORE_BF16DEV = 0, 1, 2, 3
-STORAGES = {"f64": STORE_F64, "f32": STORE_F32, "f16dev": STORE_F16DEV, "bf16dev": STORE_BF16DEV}
+STORE_F64, STORE_F32, STORE_F16DEV, STORE_BF16DEV, STORE_F16NEQ, STORE_BF16NEQ = 0, 1, 2, 3, 4, 5
+STORAGES = {"f64": STORE_F64, "f32": STORE_F32, "f16dev": STORE_F16DEV, "bf16dev": STORE_BF16DEV,
+            "f16neq": STORE_F16NEQ, "bf16neq": STORE_BF16NEQ}
 SRT_KIND, TRT_KIND, MRT_KIND = 0, 1, 2
 FORCE_NONE, FORCE_CONST, FORCE_FIELD = 0, 1, 2
 BOUNCE_BACK, MOVING_WALL = 0, 1
@@ -26,7 +27,8 @@ EXPORTS = [
     "lbm_field_create", "lbm_field_destroy", "lbm_field_info", "lbm_field_upload",
     "lbm_field_download", "lbm_field_copy",
     "lbm_model_create", "lbm_model_destroy", "lbm_model_set_srt", "lbm_model_set_trt",
-    "lbm_model_set_mrt", "lbm_model_set_force", "lbm_model_set_bcs",
+    "lbm_model_set_mrt", "lbm_model_set_force", "lbm_model_set_bcs", "lbm_model_set_launch",
+    "lbm_autotune",
     "lbm_collide", "lbm_stream", "lbm_apply_bcs", "lbm_stream_bc", "lbm_simulate",
     "lbm_simulate_timed", "lbm_moments", "lbm_reduce",
 ]
@@ -87,6 +89,8 @@ def lib() -> ctypes.CDLL:
     L.lbm_model_set_mrt.argtypes = [vp, dp, i]
     L.lbm_model_set_force.argtypes = [vp, i, dp, i, i]
     L.lbm_model_set_bcs.argtypes = [vp, ctypes.POINTER(LbmBC), i]
+    L.lbm_model_set_launch.argtypes = [vp, i, i, i]
+    L.lbm_autotune.argtypes = [vp, vp, i, ip, ip, ip, dp]
     for name in ["lbm_collide", "lbm_stream", "lbm_apply_bcs", "lbm_stream_bc"]:
         getattr(L, name).argtypes = [vp, vp, vp]
     L.lbm_simulate.argtypes = [vp, vp, vp, i64]

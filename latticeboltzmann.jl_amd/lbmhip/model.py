@@ -51,6 +51,9 @@ class LatticeBoltzmannModel:
         self.boundary_conditions = list(boundary_conditions)
         self.processing_method = processing_method
         self.delta_t = float(delta_t)
+        # launch shape of the fused step: None (automatic) or
+        # (threads per block, nodes per thread, non-temporal); see autotune()
+        self.launch = None
 
     @classmethod
     def from_populations(cls, q: Quadrature, f0: np.ndarray, collision_model: CollisionModel,
@@ -103,6 +106,7 @@ def simulate(model: LatticeBoltzmannModel, time) -> LatticeBoltzmannModel:
         m = native_model(model.quadrature).configure(model.collision_model, model.f_stream,
                                                      model.boundary_conditions,
                                                      float(steps[0]) * model.delta_t)
+        m.set_launch(model.launch)
         _lib.check(_lib.lib().lbm_simulate(m.handle, model.f_stream.handle,
                                            model.f_collision.handle, len(steps)), "simulate")
         return model
@@ -128,9 +132,29 @@ def simulate_timed(model: LatticeBoltzmannModel, nsteps: int):
 
     m = native_model(model.quadrature).configure(model.collision_model, model.f_stream,
                                                  model.boundary_conditions, 0.0)
+    m.set_launch(model.launch)
     ms = ctypes.c_double()
     nl = ctypes.c_int64()
     _lib.check(_lib.lib().lbm_simulate_timed(m.handle, model.f_stream.handle,
                                              model.f_collision.handle, int(nsteps),
                                              ctypes.byref(ms), ctypes.byref(nl)), "simulate")
     return ms.value, nl.value
+
+
+def autotune(model: LatticeBoltzmannModel, trial_steps: int = 20) -> dict:
+    """Time every launch shape of the fused step on scratch copies of the
+    model's lattice (the model state is not advanced), keep the fastest in
+    ``model.launch`` and return it with its time per step."""
+    import ctypes
+
+    m = native_model(model.quadrature).configure(model.collision_model, model.f_stream,
+                                                 model.boundary_conditions, 0.0)
+    b, v, n = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    us = ctypes.c_double()
+    _lib.check(_lib.lib().lbm_autotune(m.handle, model.f_stream.handle, int(trial_steps),
+                                       ctypes.byref(b), ctypes.byref(v), ctypes.byref(n),
+                                       ctypes.byref(us)), "lbm_autotune")
+    model.launch = (b.value, v.value, n.value)
+    m._launch_key = model.launch
+    return {"block": b.value, "nodes_per_thread": v.value, "nontemporal": n.value,
+            "us_per_step": us.value}

@@ -38,6 +38,7 @@ class NativeModel:
         self._coll_key = None
         self._force_key = None
         self._bcs_key = None
+        self._launch_key = None
 
     @property
     def handle(self):
@@ -105,6 +106,15 @@ class NativeModel:
             arr[k] = bc.to_abi()
         _lib.check(_lib.lib().lbm_model_set_bcs(self._h, arr, n), "lbm_model_set_bcs")
         self._bcs_key = key
+
+    def set_launch(self, launch) -> None:
+        """(threads per block, nodes per thread, non-temporal) of the fused
+        step; None = automatic."""
+        key = tuple(launch) if launch is not None else (0, 0, -1)
+        if key != self._launch_key:
+            _lib.check(_lib.lib().lbm_model_set_launch(self._h, *[int(k) for k in key]),
+                       "lbm_model_set_launch")
+        self._launch_key = key
 
     def configure(self, cm, f: HipLattice, bcs=None, time: float = 0.0) -> "NativeModel":
         if cm is not None:

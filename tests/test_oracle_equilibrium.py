@@ -22,13 +22,24 @@ def logspace(a, b, n):
     return 10.0 ** np.linspace(a, b, n)
 
 
+def _ok_density(rho):
+    return abs(rho - 1.0) <= 1e-8 + 1e-10
+
+
 @pytest.mark.parametrize("qid", QIDS)
 def test_mass_and_momentum(oracle_lib, qid):
+    """runtests.jl:68-117, including its fallback: where the closed form's
+    density misses (cancellation of the 4th-order terms of D2Q37 at |u| = 100)
+    the reference's test replaces f by the Hermite-sum equilibrium (:77-86)."""
     for u in logspace(2, -9, 12):
         for v in ([0.0, 0.0], [u, 0.0], [0.0, u], [u, u]):
             f = oracle_lib.equilibrium(qid, 1.0, np.array(v))
+            if u <= 1.0:
+                assert _ok_density(oracle_lib.density(qid, f)), (qid, v)
+            if not _ok_density(oracle_lib.density(qid, f)):
+                f = oracle_lib.hermite_equilibrium(qid, 1.0, np.array(v))
             rho = oracle_lib.density(qid, f)
-            assert abs(rho - 1.0) <= 1e-8 + 1e-10, (qid, v, rho)
+            assert _ok_density(rho), (qid, v, rho)
             m = oracle_lib.velocity(qid, f, 1.0)
             for d in range(2):
                 assert abs(m[d] - v[d]) <= 1e-11 + 1e-6 * abs(v[d]), (qid, v, m)
