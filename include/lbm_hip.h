@@ -112,15 +112,18 @@ int lbm_model_set_force(lbm_model m, int kind, const double *values, int nx, int
 int lbm_model_set_bcs(lbm_model m, const lbm_bc *bcs, int n);
 
 /* Launch shape of the fused step kernel: threads per block along x (64, 128,
- * 256), lattice nodes per thread along x (1, 2, 4, 8) and non-temporal
- * stores (0/1); 0 (or -1 for nontemporal) = automatic.  No counterpart in the
- * reference (a CPU loop has no launch shape). */
-int lbm_model_set_launch(lbm_model m, int block_x, int nodes_per_thread, int nontemporal);
+ * 256), lattice rows per block (1, 2, 4; block_x * rows <= 256), lattice
+ * nodes per thread along x (1, 2, 4, 8) and non-temporal stores (0/1);
+ * 0 (or -1 for nontemporal) = automatic.  No counterpart in the reference (a
+ * CPU loop has no launch shape). */
+int lbm_model_set_launch(lbm_model m, int block_x, int rows_per_block, int nodes_per_thread,
+                         int nontemporal);
 /* Autotuner: times every launch shape of the fused step for this model on
  * scratch copies of f_stream (the caller's fields are not advanced), keeps
  * the fastest in the model and returns it.  No counterpart in the reference. */
 int lbm_autotune(lbm_model m, lbm_field f_stream, int trial_steps, int *block_x,
-                 int *nodes_per_thread, int *nontemporal, double *us_per_step);
+                 int *rows_per_block, int *nodes_per_thread, int *nontemporal,
+                 double *us_per_step);
 
 /* ---- step operators ----------------------------------------------------- */
 /* collide!(cm, q, f_in, f_out; time) -- collision_models.jl:19, srt.jl:37-81,

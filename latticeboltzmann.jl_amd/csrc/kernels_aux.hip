@@ -260,6 +260,19 @@ int dispatch_reduce(int qid, int storage, const void *f, const StepParams<double
     return hipGetLastError() == hipSuccess ? 0 : -4;
 }
 
+template <int QID, int... Is>
+__device__ __forceinline__ void geq_rec_seq(const Mom4 &m, const StepParams<float> &p,
+                                            float (&geq)[QT<QID>::Q],
+                                            std::integer_sequence<int, Is...>) {
+    ((geq[Is] = geq_rec<QID, Is>(m, p)), ...);
+}
+
+template <int QID>
+__device__ __forceinline__ void geq_rec_all(const Mom4 &m, const StepParams<float> &p,
+                                            float (&geq)[QT<QID>::Q]) {
+    geq_rec_seq<QID>(m, p, geq, std::make_integer_sequence<int, QT<QID>::Q>{});
+}
+
 // Float64 (nx, ny, Q) <-> non-equilibrium storage (moments in FP32, feq in FP32
 // with the kernels' formula, 16-bit non-equilibrium parts)
 template <int QID, class ST>
@@ -285,17 +298,16 @@ k_neq_codec(double *__restrict__ hostlike, typename ST::store_t *dev, int to_dev
             my += g[i] * QT<QID>::cy(i);
         }
         const double rho = (double)pf.rho0 + drho;
-        const float r = (float)drho, ux = (float)(mx / rho), uy = (float)(my / rho);
+        const Mom4 rec = make_rec<QID>((float)drho, (float)(mx / rho), (float)(my / rho), pf);
+        m[n] = rec;
         float geq[Q];
-        feq_dev_all<QID, float>(r, ux, uy, pf, geq);
-        m[n] = Mom4{r, ux, uy, 0.0f};
+        geq_rec_all<QID>(rec, pf, geq);
 #pragma unroll
         for (int i = 0; i < Q; ++i)
             P[pf.plane * i + n] = ST::from_f((float)(g[i] - (double)geq[i]), pf);
     } else {
         float geq[Q];
-        const Mom4 mn = m[n];
-        feq_dev_all<QID, float>(mn.drho, mn.ux, mn.uy, pf, geq);
+        geq_rec_all<QID>(m[n], pf, geq);
 #pragma unroll
         for (int i = 0; i < Q; ++i)
             hostlike[pf.plane * i + n] = (double)pf.w[i] * (double)pf.rho0 + (double)geq[i] +

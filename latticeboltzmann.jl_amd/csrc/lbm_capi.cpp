@@ -218,7 +218,7 @@ struct lbm_model_s {
     size_t npart = 0;
     double *dOut = nullptr;
     // launch shape of the fused step (0 = automatic)
-    int launch_block = 0, launch_vec = 0, launch_nt = -1;
+    int launch_block = 0, launch_rows = 0, launch_vec = 0, launch_nt = -1;
 };
 
 namespace {
@@ -312,17 +312,22 @@ int max_vec(int Q, int storage) {
 }
 
 // automatic launch shape of the fused step.  Measured on MI355X, D2Q9 TRT
-// 4096^2: 256-thread blocks and non-temporal stores are fastest for every
-// storage; one node per thread for 8- and 4-byte populations, four for the
-// 2-byte storages (one dwordx2 per plane per lane).  lbm_autotune refines
-// this per lattice.
-void auto_launch(const lbm_model_s *m, const lbm_field_s *f, int &block, int &vec, int &nt) {
+// 4096^2: 256 x 1 blocks and non-temporal stores are fastest for every
+// storage; one node per thread for 8- and 4-byte populations and for the
+// non-equilibrium storages (more nodes per thread multiply the neighbour
+// moment records held in registers), four for the 16-bit deviation storages
+// (one dwordx2 per plane per lane).  lbm_autotune refines this per lattice.
+void auto_launch(const lbm_model_s *m, const lbm_field_s *f, int &block, int &rows, int &vec,
+                 int &nt) {
     block = m->launch_block > 0 ? m->launch_block : 256;
+    rows = m->launch_rows > 0 ? m->launch_rows : 1;
+    if (block * rows > 256) rows = 256 / block;
     vec = m->launch_vec;
     nt = m->launch_nt;
     if (vec <= 0) {
         vec = 1;
-        if (m->coll != LBM_MRT && storage_bytes(f->storage) == 2) {
+        const bool neq = f->storage == LBM_STORE_F16NEQ || f->storage == LBM_STORE_BF16NEQ;
+        if (m->coll != LBM_MRT && storage_bytes(f->storage) == 2 && !neq) {
             const int want = max_vec(f->Q, f->storage) < 4 ? max_vec(f->Q, f->storage) : 4;
             for (int v = want; v > 1; v >>= 1)
                 if (f->nx % v == 0 && f->nx / v >= 64) {
@@ -342,7 +347,7 @@ int run(lbm::Op op, lbm_model m, lbm_field src, lbm_field dst, bool with_bcs) {
     L.storage = src->storage;
     L.src = src->d;
     L.dst = dst->d;
-    if (op == lbm::Op::Fused) auto_launch(m, src, L.block, L.vec, L.nt);
+    if (op == lbm::Op::Fused) auto_launch(m, src, L.block, L.block_y, L.vec, L.nt);
     const lbm::StepParams<double> p = make_params(m, src, with_bcs);
     const int rc = src->storage == LBM_STORE_F64 ? lbm::dispatch_f64(L, p, g_stream)
                                                  : lbm::dispatch_lp(L, p, g_stream);
@@ -736,21 +741,28 @@ static int simulate_impl(lbm_model m, lbm_field S, lbm_field C, int64_t nsteps, 
     return 0;
 }
 
-int lbm_model_set_launch(lbm_model m, int block_x, int nodes_per_thread, int nontemporal) {
+int lbm_model_set_launch(lbm_model m, int block_x, int rows_per_block, int nodes_per_thread,
+                         int nontemporal) {
     if (!m) return fail(-1, "null model");
     if (block_x != 0 && block_x != 64 && block_x != 128 && block_x != 256)
         return fail(-1, "block_x must be 0 (automatic), 64, 128 or 256");
+    if (rows_per_block != 0 && rows_per_block != 1 && rows_per_block != 2 && rows_per_block != 4)
+        return fail(-1, "rows_per_block must be 0 (automatic), 1, 2 or 4");
+    if (block_x * (rows_per_block > 0 ? rows_per_block : 1) > 256)
+        return fail(-1, "block_x * rows_per_block must not exceed 256");
     if (nodes_per_thread != 0 && nodes_per_thread != 1 && nodes_per_thread != 2 &&
         nodes_per_thread != 4 && nodes_per_thread != 8)
         return fail(-1, "nodes_per_thread must be 0 (automatic), 1, 2, 4 or 8");
     m->launch_block = block_x;
+    m->launch_rows = rows_per_block;
     m->launch_vec = nodes_per_thread;
     m->launch_nt = nontemporal < 0 ? -1 : (nontemporal ? 1 : 0);
     return 0;
 }
 
 int lbm_autotune(lbm_model m, lbm_field f_stream, int trial_steps, int *block_x,
-                 int *nodes_per_thread, int *nontemporal, double *us_per_step) {
+                 int *rows_per_block, int *nodes_per_thread, int *nontemporal,
+                 double *us_per_step) {
     if (!m) return fail(-1, "null model");
     if (int rc = check_field(f_stream, "lbm_autotune")) return rc;
     if (f_stream->qid != m->qid) return fail(-1, "lbm_autotune: quadrature mismatch");
@@ -762,19 +774,23 @@ int lbm_autotune(lbm_model m, lbm_field f_stream, int trial_steps, int *block_x,
     if (!rc) rc = lbm_field_create(&B, f_stream->qid, f_stream->nx, f_stream->ny,
                                    f_stream->storage, f_stream->rho0);
     if (!rc) rc = lbm_field_copy(A, f_stream);
-    const int save_b = m->launch_block, save_v = m->launch_vec, save_n = m->launch_nt;
+    const int save_b = m->launch_block, save_r = m->launch_rows, save_v = m->launch_vec,
+              save_n = m->launch_nt;
     double best = 1e300;
-    int bb = 0, bv = 1, bn = 0;
+    int bb = 0, br = 1, bv = 1, bn = 0;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (!rc && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
         rc = fail(-3, "hipEventCreate failed");
     const int mv = (m->coll == LBM_MRT) ? 1 : max_vec(f_stream->Q, f_stream->storage);
-    const int blocks[3] = {64, 128, 256};
+    // (threads along x, rows) per block
+    const int shapes[6][2] = {{64, 1}, {128, 1}, {256, 1}, {64, 4}, {128, 2}, {64, 2}};
     for (int vec = 1; !rc && vec <= mv; vec <<= 1) {
         if (f_stream->nx % vec) continue;
         for (int nt = 0; !rc && nt <= (m->coll == LBM_MRT ? 0 : 1); ++nt) {
-            for (int bi = 0; !rc && bi < 3; ++bi) {
-                m->launch_block = blocks[bi];
+            for (int bi = 0; !rc && bi < 6; ++bi) {
+                if (m->coll == LBM_MRT && shapes[bi][1] != 1) continue;
+                m->launch_block = shapes[bi][0];
+                m->launch_rows = shapes[bi][1];
                 m->launch_vec = vec;
                 m->launch_nt = nt;
                 rc = run(lbm::Op::Fused, m, A, B, true);  // warm-up
@@ -792,7 +808,8 @@ int lbm_autotune(lbm_model m, lbm_field f_stream, int trial_steps, int *block_x,
                 const double us = 1e3 * (double)ms / trial_steps;
                 if (!rc && us < best) {
                     best = us;
-                    bb = blocks[bi];
+                    bb = shapes[bi][0];
+                    br = shapes[bi][1];
                     bv = vec;
                     bn = nt;
                 }
@@ -805,14 +822,17 @@ int lbm_autotune(lbm_model m, lbm_field f_stream, int trial_steps, int *block_x,
     lbm_field_destroy(B);
     if (rc) {
         m->launch_block = save_b;
+        m->launch_rows = save_r;
         m->launch_vec = save_v;
         m->launch_nt = save_n;
         return rc;
     }
     m->launch_block = bb;
+    m->launch_rows = br;
     m->launch_vec = bv;
     m->launch_nt = bn;
     if (block_x) *block_x = bb;
+    if (rows_per_block) *rows_per_block = br;
     if (nodes_per_thread) *nodes_per_thread = bv;
     if (nontemporal) *nontemporal = bn;
     if (us_per_step) *us_per_step = best;

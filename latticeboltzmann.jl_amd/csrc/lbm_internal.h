@@ -91,6 +91,7 @@ struct Launch {
     const void *src;
     void *dst;
     int block;    // threads per block along x (0 = default)
+    int block_y = 1;  // fused step: rows per block (block.x * block_y <= 256)
     int vec = 1;  // fused step: nodes per thread along x (1, 2, 4, 8)
     int nt = 0;   // fused step: non-temporal stores
 };

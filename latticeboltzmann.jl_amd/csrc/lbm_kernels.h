@@ -82,14 +82,20 @@ struct StBF16Dev : StBase<StBF16Dev, __hip_bfloat16, float> {
     }
 };
 
-// Non-equilibrium storage: per node one 16-byte record of FP32 moments
-// (rho - rho0, ux, uy, unused) of the stored population vector and, per
-// population, the 16-bit non-equilibrium part (f_i - feq_i(rho, u)) * dscale;
-// the arithmetic value is g_i = f_i - w_i rho0 = (feq_i - w_i rho0) + (f_i - feq_i).
+// Non-equilibrium storage: per node one 16-byte equilibrium record (below)
+// and, per population, the 16-bit non-equilibrium part
+// (f_i - feq_i) * dscale; the arithmetic value is
+// g_i = f_i - w_i rho0 = (feq_i - w_i rho0) + (f_i - feq_i).
 // The field holds the node records first (one dwordx4 per neighbour read),
 // then Q 16-bit planes.
+//
+// Equilibrium record.  For first- and second-order equilibria it holds
+//   a = (drho - s rho |u|^2 / 2,  s rho ux,  s rho uy,  1 / (2 rho))
+// so that feq_i - w_i rho0 = w_i (a0 + c_i.A + a3 (c_i.A)^2), A = (a1, a2):
+// six operations per population instead of the full polynomial.  For the
+// third- and fourth-order sets it holds (drho, ux, uy, 0).
 struct alignas(16) Mom4 {
-    float drho, ux, uy, pad;
+    float a0, a1, a2, a3;
 };
 
 template <class S, int ID>
@@ -222,9 +228,7 @@ __device__ __forceinline__ typename ST::T ld_pop(const typename ST::store_t *__r
                                                  const StepParams<typename ST::T> &p,
                                                  long long n) {
     if constexpr (ST::kNeq) {
-        const Mom4 m = ST::mom(F)[n];
-        const float usq = m.ux * m.ux + m.uy * m.uy;
-        return feq_dev_i<QID, I, float>(m.drho, p.rho0 + m.drho, m.ux, m.uy, usq, p) +
+        return geq_rec<QID, I>(ST::mom(F)[n], p) +
                ST::to_f(ST::pops(F, p.plane)[p.plane * I + n], p);
     } else {
         return ST::load(F + p.plane * I, n, I, p);
@@ -263,6 +267,34 @@ template <int QID, class T>
 __device__ __forceinline__ void feq_dev_all(T drho, T ux, T uy, const StepParams<T> &p,
                                             T (&geq)[QT<QID>::Q]) {
     feq_dev_all_seq<QID, T>(drho, ux, uy, p, geq, std::make_integer_sequence<int, QT<QID>::Q>{});
+}
+
+// equilibrium record of (drho, u) for quadrature QID (see Mom4)
+template <int QID>
+__device__ __forceinline__ Mom4 make_rec(float drho, float ux, float uy, const StepParams<float> &p) {
+    constexpr int FEQ = QT<QID>::FEQ;
+    if constexpr (FEQ <= 2) {
+        const float rho = p.rho0 + drho;
+        const float sr = p.s * rho;
+        const float a0 = FEQ == 2 ? drho - 0.5f * sr * (ux * ux + uy * uy) : drho;
+        return Mom4{a0, sr * ux, sr * uy, FEQ == 2 ? 0.5f / rho : 0.0f};
+    } else {
+        return Mom4{drho, ux, uy, 0.0f};
+    }
+}
+
+// feq_I - w_I rho0 from an equilibrium record
+template <int QID, int I>
+__device__ __forceinline__ float geq_rec(const Mom4 &m, const StepParams<float> &p) {
+    constexpr int FEQ = QT<QID>::FEQ;
+    if constexpr (FEQ <= 2) {
+        const float cA = udot<QID, I>(m.a1, m.a2);
+        if constexpr (FEQ == 2) return p.w[I] * ((m.a0 + cA) + m.a3 * (cA * cA));
+        else return p.w[I] * (m.a0 + cA);
+    } else {
+        const float usq = m.a1 * m.a1 + m.a2 * m.a2;
+        return feq_dev_i<QID, I, float>(m.a0, p.rho0 + m.a0, m.a1, m.a2, usq, p);
+    }
 }
 
 // shifted moments: drho = sum g, rho u = sum c g (sum_i c_i w_i = 0)
@@ -324,7 +356,8 @@ __device__ __forceinline__ void node_force(const StepParams<T> &p, int x, int y,
 // ---------------------------------------------------------------------------
 template <int QID, int COLL, bool SHIFT, class T, int... Is>
 __device__ __forceinline__ void collide_seq(T (&f)[QT<QID>::Q], const StepParams<T> &p, int x,
-                                            int y, std::integer_sequence<int, Is...>) {
+                                            int y, std::integer_sequence<int, Is...>,
+                                            Mom4 *rec_out = nullptr, T *geq_out = nullptr) {
     constexpr int Q = QT<QID>::Q;
     T rho, ux, uy, drho = T(0);
     if constexpr (SHIFT) {
@@ -339,10 +372,19 @@ __device__ __forceinline__ void collide_seq(T (&f)[QT<QID>::Q], const StepParams
         ux = ux + p.tau_force * Fx;  // srt.jl:73 / trt.jl:160
         uy = uy + p.tau_force * Fy;
     }
-    const T usq = ux * ux + uy * uy;
     T feq[Q];
-    if constexpr (SHIFT) ((feq[Is] = feq_dev_i<QID, Is, T>(drho, rho, ux, uy, usq, p)), ...);
-    else ((feq[Is] = feq_i<QID, Is, T>(rho, ux, uy, usq, p)), ...);
+    if constexpr (SHIFT) {
+        // FP32 storages: the shifted equilibrium from its record (see Mom4)
+        const Mom4 rec = make_rec<QID>(drho, ux, uy, p);
+        ((feq[Is] = geq_rec<QID, Is>(rec, p)), ...);
+        if (rec_out) {
+            *rec_out = rec;
+            ((geq_out[Is] = feq[Is]), ...);
+        }
+    } else {
+        const T usq = ux * ux + uy * uy;
+        ((feq[Is] = feq_i<QID, Is, T>(rho, ux, uy, usq, p)), ...);
+    }
     if constexpr (COLL == 0) {
         // srt.jl:76-78
         ((f[Is] = p.omo * f[Is] + p.om * feq[Is]), ...);
@@ -503,14 +545,21 @@ __device__ __forceinline__ void load_local(const typename ST::store_t *__restric
 // moments of a population vector and its non-equilibrium parts (FP32)
 // moments (rho - rho0, u) of a shifted population vector and its
 // non-equilibrium parts f_i - feq_i
-template <int QID, class T>
-__device__ __forceinline__ void neq_split(const T (&f)[QT<QID>::Q], const StepParams<T> &p,
-                                          T &drho, T &ux, T &uy, T (&g)[QT<QID>::Q]) {
-    node_moments_dev<QID, T>(f, p, drho, ux, uy);
-    T geq[QT<QID>::Q];
-    feq_dev_all<QID, T>(drho, ux, uy, p, geq);
-#pragma unroll
-    for (int i = 0; i < QT<QID>::Q; ++i) g[i] = f[i] - geq[i];
+template <int QID, int... Is>
+__device__ __forceinline__ void neq_split_seq(const float (&f)[QT<QID>::Q],
+                                              const StepParams<float> &p, Mom4 &rec,
+                                              float (&g)[QT<QID>::Q],
+                                              std::integer_sequence<int, Is...>) {
+    float drho, ux, uy;
+    node_moments_dev<QID, float>(f, p, drho, ux, uy);
+    rec = make_rec<QID>(drho, ux, uy, p);
+    ((g[Is] = f[Is] - geq_rec<QID, Is>(rec, p)), ...);
+}
+
+template <int QID>
+__device__ __forceinline__ void neq_split(const float (&f)[QT<QID>::Q], const StepParams<float> &p,
+                                          Mom4 &rec, float (&g)[QT<QID>::Q]) {
+    neq_split_seq<QID>(f, p, rec, g, std::make_integer_sequence<int, QT<QID>::Q>{});
 }
 
 template <int QID, class ST, int... Is>
@@ -521,8 +570,7 @@ __device__ __forceinline__ void store_local(typename ST::store_t *__restrict__ F
     if constexpr (ST::kNeq) {
         Mom4 m;
         float g[QT<QID>::Q];
-        neq_split<QID, float>(f, p, m.drho, m.ux, m.uy, g);
-        m.pad = 0.0f;
+        neq_split<QID>(f, p, m, g);
         ST::mom(F)[node] = m;
         auto *P = ST::pops(F, p.plane);
         ((P[p.plane * Is + node] = ST::from_f(g[Is], p)), ...);
@@ -609,13 +657,7 @@ __device__ __forceinline__ void pull_interior_vec(const typename ST::store_t *__
             __builtin_memcpy(&pk, ST::pops(C, p.plane) + p.plane * i + src, sizeof(pk));
             const Mom4 *m = ST::mom(C) + src;
 #pragma unroll
-            for (int v = 0; v < VEC; ++v) {
-                const Mom4 mv = m[v];
-                const float usq = mv.ux * mv.ux + mv.uy * mv.uy;
-                f[v][i] = feq_dev_i<QID, i, float>(mv.drho, p.rho0 + mv.drho, mv.ux, mv.uy, usq,
-                                                   p) +
-                          ST::to_f(pk.v[v], p);
-            }
+            for (int v = 0; v < VEC; ++v) f[v][i] = geq_rec<QID, i>(m[v], p) + ST::to_f(pk.v[v], p);
         } else {
             __builtin_memcpy(&pk, C + p.plane * i + src, sizeof(pk));
 #pragma unroll
@@ -637,11 +679,15 @@ __device__ __forceinline__ void put_pack(S *dst, const Pack<S, VEC> &pk) {
     }
 }
 
+// NEQ storages: rec/geq are the collision's equilibrium record and values
+// (nullptr: derive them from f)
 template <int QID, class ST, int VEC, bool NT, int... Is>
 __device__ __forceinline__ void store_vec(typename ST::store_t *__restrict__ F,
                                           const StepParams<typename ST::T> &p, long long node0,
                                           const typename ST::T (&f)[VEC][QT<QID>::Q],
-                                          std::integer_sequence<int, Is...>) {
+                                          std::integer_sequence<int, Is...>,
+                                          const Mom4 *rec = nullptr,
+                                          const float (*geq)[QT<QID>::Q] = nullptr) {
     using S = typename ST::store_t;
     if constexpr (ST::kNeq) {
         constexpr int Q = QT<QID>::Q;
@@ -649,11 +695,14 @@ __device__ __forceinline__ void store_vec(typename ST::store_t *__restrict__ F,
         Mom4 *m = ST::mom(F) + node0;
 #pragma unroll
         for (int v = 0; v < VEC; ++v) {
-            Mom4 mv;
-            neq_split<QID, float>(f[v], p, mv.drho, mv.ux, mv.uy, g[v]);
-            mv.pad = 0.0f;
             Pack<Mom4, 1> pm;
-            pm.v[0] = mv;
+            if (rec) {
+                pm.v[0] = rec[v];
+#pragma unroll
+                for (int i = 0; i < Q; ++i) g[v][i] = f[v][i] - geq[v][i];
+            } else {
+                neq_split<QID>(f[v], p, pm.v[0], g[v]);
+            }
             put_pack<NT>(m + v, pm);
         }
         auto one = [&](auto Ic) {
@@ -682,8 +731,8 @@ k_fused_v(const typename ST::store_t *__restrict__ C, typename ST::store_t *__re
           const StepParams<typename ST::T> p) {
     constexpr int M = QT<QID>::MAXC;
     const int x0 = (blockIdx.x * blockDim.x + threadIdx.x) * VEC;
-    const int y = blockIdx.y;
-    if (x0 >= p.nx) return;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;  // blockDim.y rows per block
+    if (x0 >= p.nx || y >= p.ny) return;
     typename ST::T f[VEC][QT<QID>::Q];
     const long long node0 = (long long)x0 + (long long)p.nx * y;
     const bool interior = (x0 >= M) && (x0 + VEC <= p.nx - M) && (y >= M) && (y < p.ny - M);
@@ -693,92 +742,19 @@ k_fused_v(const typename ST::store_t *__restrict__ C, typename ST::store_t *__re
 #pragma unroll
         for (int v = 0; v < VEC; ++v) pull_seq<QID, ST>(C, p, x0 + v, y, f[v], LBM_SEQ);
     }
+    if constexpr (ST::kNeq) {
+        // store f - feq with the collision's own equilibrium (no recomputation)
+        Mom4 rec[VEC];
+        float geq[VEC][QT<QID>::Q];
 #pragma unroll
-    for (int v = 0; v < VEC; ++v) collide_any<QID, COLL>(f[v], p, x0 + v, y);
-    store_vec<QID, ST, VEC, NT>(Cn, p, node0, f, LBM_SEQ);
-}
-
-// ---------------------------------------------------------------------------
-// fused step for the non-equilibrium storages with LDS staging: the block
-// (one row, blockDim.x * VEC nodes) first copies the moment records of rows
-// y - M .. y + M over its x-range plus an M-node halo into LDS (periodic
-// wrap applied there), so every neighbour's moments cost one LDS read; the
-// 16-bit non-equilibrium planes are read once each from HBM as before.
-// ---------------------------------------------------------------------------
-template <int QID, int VEC>
-constexpr int neq_tile_cols() { return 256 * VEC + 2 * QT<QID>::MAXC; }
-
-template <int QID, int VEC>
-constexpr bool neq_lds_fits() {
-    return (2 * QT<QID>::MAXC + 1) * neq_tile_cols<QID, VEC>() * (int)sizeof(Mom4) <= 64 * 1024;
-}
-
-template <int QID, class ST, int VEC, int... Is>
-__device__ __forceinline__ void pull_neq_lds(const typename ST::store_t *__restrict__ C,
-                                             const StepParams<float> &p,
-                                             const Mom4 (*tile)[neq_tile_cols<QID, VEC>()],
-                                             int xb, int x0, int y, bool interior,
-                                             float (&f)[VEC][QT<QID>::Q],
-                                             std::integer_sequence<int, Is...>) {
-    using S = typename ST::store_t;
-    constexpr int M = QT<QID>::MAXC;
-    const int nx = p.nx, ny = p.ny;
-    const long long node0 = (long long)x0 + (long long)nx * y;
-    auto one = [&](auto Ic) {
-        constexpr int i = decltype(Ic)::value;
-        constexpr int cx = QT<QID>::cx(i), cy = QT<QID>::cy(i);
-        Pack<S, VEC> pk;
-        if (interior) {
-            __builtin_memcpy(&pk, ST::pops(C, p.plane) + p.plane * i + node0 - cx - (long long)nx * cy,
-                             sizeof(pk));
-        } else {
+        for (int v = 0; v < VEC; ++v)
+            collide_seq<QID, COLL, true, float>(f[v], p, x0 + v, y, LBM_SEQ, &rec[v], geq[v]);
+        store_vec<QID, ST, VEC, NT>(Cn, p, node0, f, LBM_SEQ, rec, geq);
+    } else {
 #pragma unroll
-            for (int v = 0; v < VEC; ++v) {
-                const int sx = wrap(x0 + v - cx, nx), sy = wrap(y - cy, ny);
-                pk.v[v] = ST::pops(C, p.plane)[p.plane * i + (long long)sx + (long long)nx * sy];
-            }
-        }
-#pragma unroll
-        for (int v = 0; v < VEC; ++v) {
-            const Mom4 mv = tile[M - cy][x0 + v - xb + M - cx];
-            const float usq = mv.ux * mv.ux + mv.uy * mv.uy;
-            f[v][i] = feq_dev_i<QID, i, float>(mv.drho, p.rho0 + mv.drho, mv.ux, mv.uy, usq, p) +
-                      ST::to_f(pk.v[v], p);
-        }
-    };
-    (one(std::integral_constant<int, Is>{}), ...);
-}
-
-template <int QID, int COLL, class ST, int VEC, bool NT>
-__global__ void __launch_bounds__(256)
-k_fused_neq(const typename ST::store_t *__restrict__ C, typename ST::store_t *__restrict__ Cn,
-            const StepParams<float> p) {
-    constexpr int M = QT<QID>::MAXC;
-    constexpr int COLS = neq_tile_cols<QID, VEC>();
-    __shared__ Mom4 tile[2 * M + 1][COLS];
-    const int nx = p.nx, ny = p.ny;
-    const int y = blockIdx.y;
-    const int xb = blockIdx.x * blockDim.x * VEC;
-    const int span = blockDim.x * VEC + 2 * M;  // <= COLS
-    const Mom4 *mom = ST::mom(C);
-    for (int r = 0; r < 2 * M + 1; ++r) {
-        const long long row = (long long)nx * wrap(y - M + r, ny);
-        for (int k = threadIdx.x; k < span; k += blockDim.x)
-            tile[r][k] = mom[row + wrap(xb - M + k, nx)];
+        for (int v = 0; v < VEC; ++v) collide_any<QID, COLL>(f[v], p, x0 + v, y);
+        store_vec<QID, ST, VEC, NT>(Cn, p, node0, f, LBM_SEQ);
     }
-    __syncthreads();
-    const int x0 = xb + threadIdx.x * VEC;
-    if (x0 >= nx) return;
-    const bool interior = (x0 >= M) && (x0 + VEC <= nx - M) && (y >= M) && (y < ny - M);
-    float f[VEC][QT<QID>::Q];
-    pull_neq_lds<QID, ST, VEC>(C, p, tile, xb, x0, y, interior, f, LBM_SEQ);
-    if (!interior && p.nbc > 0) {
-#pragma unroll
-        for (int v = 0; v < VEC; ++v) apply_bc_seq<QID, ST>(C, p, x0 + v, y, f[v], LBM_SEQ);
-    }
-#pragma unroll
-    for (int v = 0; v < VEC; ++v) collide_any<QID, COLL>(f[v], p, x0 + v, y);
-    store_vec<QID, ST, VEC, NT>(Cn, p, (long long)x0 + (long long)nx * y, f, LBM_SEQ);
 }
 
 // largest nodes-per-thread worth instantiating (keeps VEC * Q registers bounded)
@@ -880,16 +856,10 @@ template <int QID, int COLL, class ST, int VEC>
 int launch_fused_vn(const Launch &L, const StepParams<typename ST::T> &p, hipStream_t s) {
     using S = typename ST::store_t;
     const int bx = L.block > 0 ? L.block : default_block(p.nx / VEC);
-    const dim3 grid((p.nx / VEC + bx - 1) / bx, p.ny, 1), block(bx, 1, 1);
+    const int by = (L.block_y > 0 && bx * L.block_y <= 256) ? L.block_y : 1;
+    const dim3 grid((p.nx / VEC + bx - 1) / bx, (p.ny + by - 1) / by, 1), block(bx, by, 1);
     const S *src = static_cast<const S *>(L.src);
     S *dst = static_cast<S *>(L.dst);
-    if constexpr (ST::kNeq) {
-        if constexpr (neq_lds_fits<QID, VEC>()) {
-            if (L.nt) k_fused_neq<QID, COLL, ST, VEC, true><<<grid, block, 0, s>>>(src, dst, p);
-            else k_fused_neq<QID, COLL, ST, VEC, false><<<grid, block, 0, s>>>(src, dst, p);
-            return hipGetLastError() == hipSuccess ? 0 : -4;
-        }
-    }
     if (L.nt) k_fused_v<QID, COLL, ST, VEC, true><<<grid, block, 0, s>>>(src, dst, p);
     else k_fused_v<QID, COLL, ST, VEC, false><<<grid, block, 0, s>>>(src, dst, p);
     return hipGetLastError() == hipSuccess ? 0 : -4;

@@ -89,8 +89,8 @@ def lib() -> ctypes.CDLL:
     L.lbm_model_set_mrt.argtypes = [vp, dp, i]
     L.lbm_model_set_force.argtypes = [vp, i, dp, i, i]
     L.lbm_model_set_bcs.argtypes = [vp, ctypes.POINTER(LbmBC), i]
-    L.lbm_model_set_launch.argtypes = [vp, i, i, i]
-    L.lbm_autotune.argtypes = [vp, vp, i, ip, ip, ip, dp]
+    L.lbm_model_set_launch.argtypes = [vp, i, i, i, i]
+    L.lbm_autotune.argtypes = [vp, vp, i, ip, ip, ip, ip, dp]
     for name in ["lbm_collide", "lbm_stream", "lbm_apply_bcs", "lbm_stream_bc"]:
         getattr(L, name).argtypes = [vp, vp, vp]
     L.lbm_simulate.argtypes = [vp, vp, vp, i64]

@@ -51,8 +51,9 @@ class LatticeBoltzmannModel:
         self.boundary_conditions = list(boundary_conditions)
         self.processing_method = processing_method
         self.delta_t = float(delta_t)
-        # launch shape of the fused step: None (automatic) or
-        # (threads per block, nodes per thread, non-temporal); see autotune()
+        # launch shape of the fused step: None (automatic) or (threads per
+        # block along x, rows per block, nodes per thread, non-temporal);
+        # see autotune()
         self.launch = None
 
     @classmethod
@@ -149,12 +150,12 @@ def autotune(model: LatticeBoltzmannModel, trial_steps: int = 20) -> dict:
 
     m = native_model(model.quadrature).configure(model.collision_model, model.f_stream,
                                                  model.boundary_conditions, 0.0)
-    b, v, n = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    b, r, v, n = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
     us = ctypes.c_double()
     _lib.check(_lib.lib().lbm_autotune(m.handle, model.f_stream.handle, int(trial_steps),
-                                       ctypes.byref(b), ctypes.byref(v), ctypes.byref(n),
-                                       ctypes.byref(us)), "lbm_autotune")
-    model.launch = (b.value, v.value, n.value)
+                                       ctypes.byref(b), ctypes.byref(r), ctypes.byref(v),
+                                       ctypes.byref(n), ctypes.byref(us)), "lbm_autotune")
+    model.launch = (b.value, r.value, v.value, n.value)
     m._launch_key = model.launch
-    return {"block": b.value, "nodes_per_thread": v.value, "nontemporal": n.value,
-            "us_per_step": us.value}
+    return {"block": b.value, "rows_per_block": r.value, "nodes_per_thread": v.value,
+            "nontemporal": n.value, "us_per_step": us.value}

@@ -108,9 +108,12 @@ class NativeModel:
         self._bcs_key = key
 
     def set_launch(self, launch) -> None:
-        """(threads per block, nodes per thread, non-temporal) of the fused
-        step; None = automatic."""
-        key = tuple(launch) if launch is not None else (0, 0, -1)
+        """(threads per block along x, rows per block, nodes per thread,
+        non-temporal) of the fused step; None = automatic.  A 3-tuple omits
+        the rows (one row per block)."""
+        if launch is not None and len(launch) == 3:
+            launch = (launch[0], 1, launch[1], launch[2])
+        key = tuple(launch) if launch is not None else (0, 0, 0, -1)
         if key != self._launch_key:
             _lib.check(_lib.lib().lbm_model_set_launch(self._h, *[int(k) for k in key]),
                        "lbm_model_set_launch")
