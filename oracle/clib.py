@@ -82,6 +82,8 @@ def lib() -> ctypes.CDLL:
         _lib.lbo_collide_mrt.argtypes = [qp, i, i, dp, dp, dp, i, i, dp]
         _lib.lbo_stream.argtypes = [qp, i, i, dp, dp]
         _lib.lbo_apply_bcs.argtypes = [qp, i, i, bp, i, dp, dp]
+        _lib.lbo_apply_bcs.restype = i
+        _lib.lbo_simulate.restype = i
         _lib.lbo_simulate.argtypes = [qp, i, i, dp, dp, i, dp, i, i, dp, bp, i, ctypes.c_long]
         _lib.lbo_moments.argtypes = [qp, i, i, dp, d, dp, dp, dp, dp, dp, dp]
         _lib.lbo_set_threads.argtypes = [i]
@@ -221,7 +223,8 @@ def apply_bcs(qid, bcs, fnew, fold):
     _check_f(fold, q.Q)
     nx, ny, _ = fnew.shape
     arr = make_bcs(bcs)
-    lib().lbo_apply_bcs(ctypes.byref(q), nx, ny, arr, len(bcs), _dp(fnew), _dp(fold))
+    if lib().lbo_apply_bcs(ctypes.byref(q), nx, ny, arr, len(bcs), _dp(fnew), _dp(fold)) != 0:
+        raise IndexError("boundary range outside the lattice (BoundsError in the reference)")
 
 
 def simulate(qid, f_stream, f_collision, collision, taus, nsteps, force=None, bcs=()):
@@ -233,8 +236,9 @@ def simulate(qid, f_stream, f_collision, collision, taus, nsteps, force=None, bc
     t = np.ascontiguousarray(taus, dtype=np.float64)
     k, p, _keep = _force_args(force, nx, ny)
     arr = make_bcs(list(bcs))
-    lib().lbo_simulate(ctypes.byref(q), nx, ny, _dp(f_stream), _dp(f_collision), collision,
-                       _dp(t), t.size, k, p, arr, len(bcs), int(nsteps))
+    if lib().lbo_simulate(ctypes.byref(q), nx, ny, _dp(f_stream), _dp(f_collision), collision,
+                          _dp(t), t.size, k, p, arr, len(bcs), int(nsteps)) != 0:
+        raise IndexError("boundary range outside the lattice (BoundsError in the reference)")
 
 
 def moments(qid, f, tau):

@@ -494,8 +494,17 @@ void lbo_stream(const lbo_quad *q, int nx, int ny, const double *f, double *fnew
 /* boundary_conditions/bounce_back.jl:117-181; MovingWall{North}             */
 /* boundary_conditions/moving_wall.jl:198-219                                */
 /* ------------------------------------------------------------------------ */
-void lbo_apply_bcs(const lbo_quad *q, int nx, int ny, const lbo_bc *bcs, int nbc,
-                   double *fnew, const double *fold) {
+int lbo_apply_bcs(const lbo_quad *q, int nx, int ny, const lbo_bc *bcs, int nbc,
+                  double *fnew, const double *fold) {
+    /* the reference indexes f_new[x_idx, y_idx, :] without @inbounds: a range
+     * outside the lattice is a BoundsError, reported here as -1 */
+    for (int b = 0; b < nbc; ++b) {
+        const lbo_bc *bc = &bcs[b];
+        if (bc->kind != LBO_BOUNCE_BACK) continue;
+        if ((bc->x1 >= bc->x0 && (bc->x0 < 1 || bc->x1 > nx)) ||
+            (bc->y1 >= bc->y0 && (bc->y0 < 1 || bc->y1 > ny)))
+            return -1;
+    }
     for (int b = 0; b < nbc; ++b) {
         const lbo_bc *bc = &bcs[b];
         if (bc->kind == LBO_BOUNCE_BACK) {
@@ -535,15 +544,16 @@ void lbo_apply_bcs(const lbo_quad *q, int nx, int ny, const lbo_bc *bcs, int nbc
             }
         }
     }
+    return 0;
 }
 
 /* ------------------------------------------------------------------------ */
 /* simulate(model, time) step loop -- src/lattice_boltzmann_model.jl:60-77   */
 /* collide f_stream -> f_collision; stream f_collision -> f_stream; BCs      */
 /* ------------------------------------------------------------------------ */
-void lbo_simulate(const lbo_quad *q, int nx, int ny, double *f_stream, double *f_collision,
-                  int collision, const double *taus, int ntaus, int force_kind,
-                  const double *force, const lbo_bc *bcs, int nbc, long nsteps) {
+int lbo_simulate(const lbo_quad *q, int nx, int ny, double *f_stream, double *f_collision,
+                 int collision, const double *taus, int ntaus, int force_kind,
+                 const double *force, const lbo_bc *bcs, int nbc, long nsteps) {
     for (long t = 0; t < nsteps; ++t) {
         if (collision == 0)
             lbo_collide_srt(q, nx, ny, f_stream, f_collision, taus[0], force_kind, force);
@@ -553,8 +563,9 @@ void lbo_simulate(const lbo_quad *q, int nx, int ny, double *f_stream, double *f
         else
             lbo_collide_mrt(q, nx, ny, f_stream, f_collision, taus, ntaus, force_kind, force);
         lbo_stream(q, nx, ny, f_collision, f_stream);
-        lbo_apply_bcs(q, nx, ny, bcs, nbc, f_stream, f_collision);
+        if (lbo_apply_bcs(q, nx, ny, bcs, nbc, f_stream, f_collision) != 0) return -1;
     }
+    return 0;
 }
 
 /* ------------------------------------------------------------------------ */

@@ -60,13 +60,14 @@ void lbo_collide_trt(const lbo_quad *q, int nx, int ny, const double *fin, doubl
 void lbo_collide_mrt(const lbo_quad *q, int nx, int ny, const double *fin, double *fout,
                      const double *taus, int ntaus, int force_kind, const double *force);
 void lbo_stream(const lbo_quad *q, int nx, int ny, const double *f, double *fnew);
-void lbo_apply_bcs(const lbo_quad *q, int nx, int ny, const lbo_bc *bcs, int nbc,
-                   double *fnew, const double *fold);
+/* returns -1 (and writes nothing) when a bounce-back range leaves the lattice */
+int lbo_apply_bcs(const lbo_quad *q, int nx, int ny, const lbo_bc *bcs, int nbc,
+                  double *fnew, const double *fold);
 
 /* collision kind for lbo_simulate: 0 SRT (taus[0]), 1 TRT (taus[0]=tau_s, taus[1]=tau_a), 2 MRT */
-void lbo_simulate(const lbo_quad *q, int nx, int ny, double *f_stream, double *f_collision,
-                  int collision, const double *taus, int ntaus, int force_kind,
-                  const double *force, const lbo_bc *bcs, int nbc, long nsteps);
+int lbo_simulate(const lbo_quad *q, int nx, int ny, double *f_stream, double *f_collision,
+                 int collision, const double *taus, int ntaus, int force_kind,
+                 const double *force, const lbo_bc *bcs, int nbc, long nsteps);
 
 /* moments: rho, ux, uy (nx*ny) and deviatoric tensor sxx, sxy, syy (nx*ny); any may be NULL */
 void lbo_moments(const lbo_quad *q, int nx, int ny, const double *f, double tau,

@@ -36,8 +36,9 @@ def bc_dicts(name: str, nx: int, ny: int, u_wall=(0.05, 0.0)):
                 dict(kind=0, direction=3, **full),
                 dict(kind=1, direction=0, u=u_wall, rho=1.0, **full)]
     if name == "partial":      # ranges narrower than the lattice, list order matters
-        return [dict(kind=0, direction=3, xs=(1, nx), ys=(2, max(2, ny - 1))),
-                dict(kind=0, direction=0, xs=(2, max(2, nx - 1)), ys=(1, ny)),
+        lo_y, lo_x = min(2, ny), min(2, nx)
+        return [dict(kind=0, direction=3, xs=(1, nx), ys=(lo_y, max(lo_y, ny - 1))),
+                dict(kind=0, direction=0, xs=(lo_x, max(lo_x, nx - 1)), ys=(1, ny)),
                 dict(kind=0, direction=1, xs=(1, nx), ys=(1, ny)),
                 dict(kind=0, direction=2, xs=(1, nx), ys=(1, ny))]
     raise ValueError(name)
