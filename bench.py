@@ -83,6 +83,56 @@ def cpu_baseline(n, tau_s, tau_a, f0, target_s):
                       f"({dt:.1f} s) with the C restatement on {threads} OpenMP threads"}
 
 
+def shear_wave_populations(L, q, n, amp, k):
+    y = np.arange(n)
+    ux = np.tile(amp * np.sin(2 * np.pi * k * y / n), (n, 1))
+    return L.equilibrium_populations(q, np.ones((n, n)), ux, np.zeros((n, n)))
+
+
+def batch_leg(L, steps=100, count=512, seed=0):
+    """512 independent periodic D2Q9 TRT lattices (sizes log-uniform in
+    32^2..1024^2, tau ~ U(0.52, 1.5), shear waves of amplitude U(0.001, 0.05)
+    and wavenumber 1, 2 or 4; NumPy PCG64 seed 0), FP32 storage: one batched
+    launch per step with 100-step hipGraph chunks, against the same lattices
+    advanced one after another with per-lattice launches."""
+    rng = np.random.default_rng(seed)
+    sizes = np.round(np.exp(rng.uniform(np.log(32), np.log(1024), count))).astype(int)
+    tau = rng.uniform(0.52, 1.5, count)
+    amp = rng.uniform(0.001, 0.05, count)
+    kk = rng.choice([1, 2, 4], count)
+    tau_a = 0.5 + 0.25 / (tau - 0.5)
+    q = L.D2Q9()
+    b = L.LatticeBatch(q, [(n, n) for n in sizes], tau, tau_a, storage="f32")
+    for k in range(count):
+        b.upload(k, shear_wave_populations(L, q, int(sizes[k]), amp[k], kk[k]))
+    tuned = b.autotune(3)
+    b.simulate(steps, graph_chunk=steps)                      # warm-up (builds the graph)
+    L.synchronize()
+    t0 = time.perf_counter()
+    fused_ms = b.simulate(steps, graph_chunk=steps)
+    wall_b = time.perf_counter() - t0
+    nodes = int(np.sum(sizes.astype(np.int64) ** 2))
+    seq = 0.0
+    for k in range(count):
+        f0 = shear_wave_populations(L, q, int(sizes[k]), amp[k], kk[k])
+        m = L.LatticeBoltzmannModel.from_populations(q, f0, L.TRT.fields(tau[k], tau_a[k]),
+                                                     storage="f32")
+        L.simulate(m, range(2))
+        L.synchronize()
+        t0 = time.perf_counter()
+        L.simulate(m, range(steps))
+        L.synchronize()
+        seq += time.perf_counter() - t0
+        del m
+    del b
+    return {"lattices": count, "nodes": nodes, "steps": steps, "storage": "f32",
+            "graph_chunk": steps, "launch": tuned,
+            "batched_mlups": round(nodes * steps / wall_b / 1e6, 1),
+            "batched_fused_kernel_ms": round(fused_ms, 3),
+            "sequential_mlups": round(nodes * steps / seq / 1e6, 1),
+            "speedup_vs_sequential": round(seq / wall_b, 2)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -93,6 +143,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp16", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
@@ -187,6 +238,9 @@ def main():
             del m16
 
     del m64
+    if not a.no_batch:
+        print("[bench] batch of 512 lattices", file=sys.stderr, flush=True)
+        result["batch"] = batch_leg(L)
     if not a.no_cpu:
         print("[bench] CPU baseline (oracle restatement)", file=sys.stderr, flush=True)
         result["cpu_baseline"] = cpu_baseline(n, cm.τ_symmetric, cm.τ_asymmetric, f0,

@@ -146,6 +146,34 @@ int lbm_simulate(lbm_model m, lbm_field f_stream, lbm_field f_collision, int64_t
 int lbm_simulate_timed(lbm_model m, lbm_field f_stream, lbm_field f_collision, int64_t nsteps,
                        double *ms_total, int64_t *nlaunch);
 
+/* ---- batches of independent lattices ----------------------------------- */
+/* Many independent periodic lattices (uneven sizes and relaxation times)
+ * advanced together: every time step is one persistent kernel launch over a
+ * tile table covering all lattices, and runs of fused steps are replayed
+ * from a captured hipGraph.  D2Q9, storage LBM_STORE_F64 or LBM_STORE_F32.
+ * No counterpart in the reference (it advances one lattice per process); each
+ * lattice follows simulate(model, time) exactly as lbm_simulate does. */
+typedef struct lbm_batch_s *lbm_batch;
+/* tau_a == NULL: SRT(tau_s[k]); otherwise TRT(tau_s[k], tau_a[k]) */
+int lbm_batch_create(lbm_batch *out, int quadrature, int storage, int n, const int *nx,
+                     const int *ny, const double *tau_s, const double *tau_a, double rho0);
+int lbm_batch_destroy(lbm_batch b);
+int lbm_batch_info(lbm_batch b, int *n, int64_t *nodes, int64_t *bytes);
+/* populations of lattice k, (nx_k, ny_k, Q) column-major Float64: upload sets
+ * both f_stream and f_collision (f_collision = copy(f_stream)); role 0 =
+ * f_stream, 1 = f_collision */
+int lbm_batch_upload(lbm_batch b, int k, const double *f);
+int lbm_batch_download(lbm_batch b, int k, int role, double *f);
+/* threads per tile-block (64, 128, 256) and persistent grid size (0 = one
+ * resident wave of blocks per CU slot) */
+int lbm_batch_set_launch(lbm_batch b, int block, int grid);
+/* times launch shapes on the batch (state saved and restored), keeps the fastest */
+int lbm_batch_autotune(lbm_batch b, int trial_steps, int *block, int *grid, double *us_per_step);
+/* nsteps of every lattice; graph_chunk > 0 replays runs of graph_chunk fused
+ * steps from one captured hipGraph (0: direct launches).  *ms_fused (may be
+ * NULL) receives the time of the fused steps (HIP events). */
+int lbm_batch_simulate(lbm_batch b, int64_t nsteps, int graph_chunk, double *ms_fused);
+
 /* ---- moments ------------------------------------------------------------ */
 /* density / velocity! / deviatoric_tensor per node (moments.jl:1-19,199-214),
  * as TrackHydrodynamicErrors uses them (track_hydrodynamic_errors.jl:139-177).

@@ -49,7 +49,8 @@ def _units():
              ("lbm_dispatch.cpp", ["-x", "hip", "-ffp-contract=off"])]
     for name in sorted(os.listdir(CSRC)):
         if name.startswith("k_") and name.endswith(".hip"):
-            contract = "off" if name.startswith("k_f64_") else "fast"
+            contract = "off" if (name.startswith("k_f64_") or name.startswith("k_batch_f64")) \
+                else "fast"
             units.append((name, [f"-ffp-contract={contract}"]))
     return units
 

@@ -365,6 +365,11 @@ void free_dev(void *&p) {
 
 }  // namespace
 
+static int lbm_field_copy_raw(void *dst, const void *src, size_t bytes) {
+    if (int rc = lbm::dispatch_copy(src, dst, bytes, g_stream)) return fail(-4, "copy kernel failed (%d)", rc);
+    return 0;
+}
+
 extern "C" {
 
 const char *lbm_last_error(void) { return g_err.c_str(); }
@@ -924,6 +929,366 @@ int lbm_reduce(lbm_field f, double *out5) {
     free_dev(reinterpret_cast<void *&>(ftmp));
     if (e != hipSuccess) return fail(-3, "lbm_reduce: %s", hipGetErrorString(e));
     if (rc) return fail(-4, "reduction kernel failed (%d)", rc);
+    return 0;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// batches of independent lattices
+// ---------------------------------------------------------------------------
+struct lbm_batch_s {
+    int qid = 9, storage = LBM_STORE_F32, n = 0;
+    double rho0 = 1.0;
+    HostQuad q;
+    std::vector<lbm::BatchLattice> lats;
+    std::vector<int> nx, ny;
+    lbm::BatchLattice *dLats = nullptr;
+    lbm::BatchTile *dTiles = nullptr;
+    int ntiles = 0, tile_block = 0;
+    void *pool = nullptr;
+    size_t pool_bytes = 0;
+    int64_t nodes = 0;
+    int parity_stream = 0;  // buffer holding f_stream
+    int block = 256, grid = 0;
+    hipGraphExec_t gexec = nullptr;
+    hipGraph_t graph = nullptr;
+    int g_chunk = 0, g_parity = -1, g_block = 0, g_grid = 0;
+};
+
+namespace {
+
+int batch_elem(const lbm_batch_s *b) { return storage_bytes(b->storage); }
+
+int batch_tiles(lbm_batch b, int block) {
+    if (b->tile_block == block && b->dTiles) return 0;
+    std::vector<lbm::BatchTile> tiles;
+    for (int k = 0; k < b->n; ++k)
+        for (long long n0 = 0; n0 < b->lats[k].plane; n0 += block)
+            tiles.push_back(lbm::BatchTile{k, (int)n0});
+    free_dev(reinterpret_cast<void *&>(b->dTiles));
+    HIP_TRY(hipMalloc(&b->dTiles, tiles.size() * sizeof(lbm::BatchTile)));
+    HIP_TRY(hipMemcpyAsync(b->dTiles, tiles.data(), tiles.size() * sizeof(lbm::BatchTile),
+                           hipMemcpyHostToDevice, g_stream));
+    HIP_TRY(hipStreamSynchronize(g_stream));
+    b->ntiles = (int)tiles.size();
+    b->tile_block = block;
+    return 0;
+}
+
+int batch_grid(const lbm_batch_s *b, int block) {
+    if (b->grid > 0) return b->grid;
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) == hipSuccess)
+        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    const int per_cu = 2048 / block;  // one resident wave of blocks
+    const long long g = (long long)cus * per_cu;
+    return (int)(g < b->ntiles ? g : b->ntiles);
+}
+
+lbm::StepParams<double> batch_params(const lbm_batch_s *b) {
+    lbm::StepParams<double> p{};
+    const HostQuad &q = b->q;
+    for (int i = 0; i < 37; ++i) p.w[i] = i < q.Q ? q.w[i] : 0.0;
+    const double s = q.s;
+    p.s = s;
+    p.neg_s = -s;
+    p.s2 = s * s;
+    p.three_s = 3.0 * s;
+    p.six_s3 = 6.0 * ((s * s) * s);
+    p.three_s2 = 3.0 * (s * s);
+    p.s4 = pow4(s);
+    p.cs_inv = 1.0 / s;
+    p.rho0 = b->rho0;
+    p.dscale = p.dscale_inv = 1.0;
+    return p;
+}
+
+int batch_launch(lbm_batch b, int op, int parity) {
+    if (int rc = batch_tiles(b, b->block)) return rc;
+    const int grid = batch_grid(b, b->block);
+    const lbm::StepParams<double> p = batch_params(b);
+    int rc;
+    if (b->storage == LBM_STORE_F64)
+        rc = lbm::run_batch_f64_q9(op, b->dLats, b->dTiles, b->ntiles, b->pool, parity, b->block,
+                                   grid, p, g_stream);
+    else
+        rc = lbm::run_batch_f32_q9(op, b->dLats, b->dTiles, b->ntiles, b->pool, parity, b->block,
+                                   grid, lbm::to_float_params(p), g_stream);
+    if (rc) return fail(-4, "batch kernel launch failed (%d)", rc);
+    return 0;
+}
+
+void batch_drop_graph(lbm_batch b) {
+    if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->gexec = nullptr;
+    b->graph = nullptr;
+    b->g_chunk = 0;
+    b->g_parity = -1;
+}
+
+// capture `chunk` fused steps starting at `parity` (chunk even: parity returns)
+int batch_graph(lbm_batch b, int chunk, int parity) {
+    const int grid = batch_grid(b, b->block);
+    if (b->gexec && b->g_chunk == chunk && b->g_parity == parity && b->g_block == b->block &&
+        b->g_grid == grid)
+        return 0;
+    batch_drop_graph(b);
+    if (int rc = batch_tiles(b, b->block)) return rc;
+    HIP_TRY(hipStreamBeginCapture(g_stream, hipStreamCaptureModeThreadLocal));
+    int rc = 0, cur = parity;
+    for (int k = 0; k < chunk && !rc; ++k) {
+        rc = batch_launch(b, 1, cur);
+        cur ^= 1;
+    }
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(g_stream, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return fail(-3, "hipStreamEndCapture: %s", hipGetErrorString(e));
+    e = hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        return fail(-3, "hipGraphInstantiate: %s", hipGetErrorString(e));
+    }
+    b->graph = g;
+    b->g_chunk = chunk;
+    b->g_parity = parity;
+    b->g_block = b->block;
+    b->g_grid = grid;
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lbm_batch_create(lbm_batch *out, int quadrature, int storage, int n, const int *nx,
+                     const int *ny, const double *tau_s, const double *tau_a, double rho0) {
+    if (!out) return fail(-1, "lbm_batch_create: null output handle");
+    *out = nullptr;
+    if (quadrature != LBM_D2Q9) return fail(-2, "batches are implemented for D2Q9");
+    if (storage != LBM_STORE_F64 && storage != LBM_STORE_F32)
+        return fail(-2, "batches use LBM_STORE_F64 or LBM_STORE_F32");
+    if (n < 1 || !nx || !ny || !tau_s) return fail(-1, "lbm_batch_create: empty batch");
+    if (int rc = ensure_stream()) return rc;
+    lbm_batch b = new lbm_batch_s;
+    b->qid = quadrature;
+    b->storage = storage;
+    b->n = n;
+    b->rho0 = rho0;
+    host_quad(quadrature, b->q);
+    const int eb = storage_bytes(storage);
+    long long off = 0;
+    for (int k = 0; k < n; ++k) {
+        if (nx[k] < 1 || ny[k] < 1 || (long long)nx[k] * ny[k] > (1LL << 30)) {
+            delete b;
+            return fail(-1, "lattice %d: size %dx%d not supported", k, nx[k], ny[k]);
+        }
+        const double ts = tau_s[k], ta = tau_a ? tau_a[k] : tau_s[k];
+        if (!(ts > 0.0) || !(ta > 0.0)) {
+            delete b;
+            return fail(-1, "lattice %d: relaxation times must be positive", k);
+        }
+        lbm::BatchLattice L{};
+        L.nx = nx[k];
+        L.ny = ny[k];
+        L.plane = (long long)nx[k] * ny[k];
+        L.off[0] = off;
+        off += L.plane * b->q.Q;
+        L.off[1] = off;
+        off += L.plane * b->q.Q;
+        L.omo = 1.0 - 1.0 / ts;
+        L.om = 1.0 / ts;
+        L.oms = 1.0 / ts;
+        L.oma = 1.0 / ta;
+        L.coll = tau_a ? 1 : 0;
+        b->lats.push_back(L);
+        b->nx.push_back(nx[k]);
+        b->ny.push_back(ny[k]);
+        b->nodes += L.plane;
+    }
+    b->pool_bytes = (size_t)off * eb;
+    hipError_t e = hipMalloc(&b->pool, b->pool_bytes);
+    if (e == hipSuccess) e = hipMalloc(&b->dLats, b->lats.size() * sizeof(lbm::BatchLattice));
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(b->dLats, b->lats.data(), b->lats.size() * sizeof(lbm::BatchLattice),
+                           hipMemcpyHostToDevice, g_stream);
+    if (e == hipSuccess) e = hipMemsetAsync(b->pool, 0, b->pool_bytes, g_stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(g_stream);
+    if (e != hipSuccess) {
+        free_dev(b->pool);
+        free_dev(reinterpret_cast<void *&>(b->dLats));
+        delete b;
+        return fail(-3, "batch allocation failed: %s", hipGetErrorString(e));
+    }
+    *out = b;
+    return 0;
+}
+
+int lbm_batch_destroy(lbm_batch b) {
+    if (!b) return 0;
+    if (g_stream) (void)hipStreamSynchronize(g_stream);
+    batch_drop_graph(b);
+    free_dev(b->pool);
+    free_dev(reinterpret_cast<void *&>(b->dLats));
+    free_dev(reinterpret_cast<void *&>(b->dTiles));
+    delete b;
+    return 0;
+}
+
+int lbm_batch_info(lbm_batch b, int *n, int64_t *nodes, int64_t *bytes) {
+    if (!b) return fail(-1, "null batch");
+    if (n) *n = b->n;
+    if (nodes) *nodes = b->nodes;
+    if (bytes) *bytes = (int64_t)b->pool_bytes;
+    return 0;
+}
+
+static int batch_xfer(lbm_batch b, int k, int buf, double *host, bool up) {
+    if (!b) return fail(-1, "null batch");
+    if (k < 0 || k >= b->n) return fail(-1, "lattice index %d outside 0..%d", k, b->n - 1);
+    if (!host) return fail(-1, "null host array");
+    const lbm::BatchLattice &L = b->lats[k];
+    const size_t nb64 = (size_t)L.plane * b->q.Q * sizeof(double);
+    const int eb = batch_elem(b);
+    char *dst = static_cast<char *>(b->pool) + (size_t)L.off[buf] * eb;
+    if (b->storage == LBM_STORE_F64) {
+        HIP_TRY(hipMemcpyAsync(up ? (void *)dst : (void *)host, up ? (const void *)host : dst, nb64,
+                               up ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost, g_stream));
+        HIP_TRY(hipStreamSynchronize(g_stream));
+        return 0;
+    }
+    double *tmp = nullptr;
+    HIP_TRY(hipMalloc(&tmp, nb64));
+    lbm::StepParams<double> p = batch_params(b);
+    p.nx = L.nx;
+    p.ny = L.ny;
+    p.plane = L.plane;
+    hipError_t e = hipSuccess;
+    if (up) e = hipMemcpyAsync(tmp, host, nb64, hipMemcpyHostToDevice, g_stream);
+    int rc = 0;
+    if (e == hipSuccess) rc = lbm::dispatch_upload(b->qid, b->storage, tmp, dst, p, up ? 1 : 0, g_stream);
+    if (e == hipSuccess && !rc && !up) e = hipMemcpyAsync(host, tmp, nb64, hipMemcpyDeviceToHost, g_stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(g_stream);
+    (void)hipFree(tmp);
+    if (e != hipSuccess) return fail(-3, "batch transfer failed: %s", hipGetErrorString(e));
+    if (rc) return fail(-4, "conversion kernel failed (%d)", rc);
+    return 0;
+}
+
+int lbm_batch_upload(lbm_batch b, int k, const double *f) {
+    if (int rc = batch_xfer(b, k, 0, const_cast<double *>(f), true)) return rc;
+    if (int rc = batch_xfer(b, k, 1, const_cast<double *>(f), true)) return rc;
+    return 0;
+}
+
+int lbm_batch_download(lbm_batch b, int k, int role, double *f) {
+    if (!b) return fail(-1, "null batch");
+    if (role != 0 && role != 1) return fail(-1, "role must be 0 (f_stream) or 1 (f_collision)");
+    const int buf = role == 0 ? b->parity_stream : (b->parity_stream ^ 1);
+    return batch_xfer(b, k, buf, f, false);
+}
+
+int lbm_batch_set_launch(lbm_batch b, int block, int grid) {
+    if (!b) return fail(-1, "null batch");
+    if (block != 0 && block != 64 && block != 128 && block != 256)
+        return fail(-1, "block must be 0 (automatic), 64, 128 or 256");
+    if (grid < 0) return fail(-1, "grid must be >= 0");
+    b->block = block ? block : 256;
+    b->grid = grid;
+    return 0;
+}
+
+int lbm_batch_simulate(lbm_batch b, int64_t nsteps, int graph_chunk, double *ms_fused) {
+    if (!b) return fail(-1, "null batch");
+    if (nsteps < 0 || graph_chunk < 0) return fail(-1, "negative step count");
+    if (ms_fused) *ms_fused = 0.0;
+    if (nsteps == 0) return 0;
+    const int chunk = graph_chunk & ~1;  // even: the parity returns after a replay
+    if (int rc = batch_launch(b, 0, b->parity_stream)) return rc;  // C_0 = collide(S_0)
+    int cur = b->parity_stream ^ 1;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (ms_fused) {
+        HIP_TRY(hipEventCreate(&e0));
+        HIP_TRY(hipEventCreate(&e1));
+        HIP_TRY(hipEventRecord(e0, g_stream));
+    }
+    int64_t left = nsteps - 1;
+    if (chunk > 0 && left >= chunk) {
+        if (int rc = batch_graph(b, chunk, cur)) return rc;
+        while (left >= chunk) {
+            HIP_TRY(hipGraphLaunch(b->gexec, g_stream));
+            left -= chunk;
+        }
+    }
+    for (; left > 0; --left) {
+        if (int rc = batch_launch(b, 1, cur)) return rc;
+        cur ^= 1;
+    }
+    if (ms_fused) HIP_TRY(hipEventRecord(e1, g_stream));
+    if (int rc = batch_launch(b, 2, cur)) return rc;  // S_n = stream(C_{n-1})
+    b->parity_stream = cur ^ 1;
+    if (ms_fused) {
+        HIP_TRY(hipEventSynchronize(e1));
+        float t = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&t, e0, e1));
+        *ms_fused = (double)t;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+    }
+    HIP_TRY(hipStreamSynchronize(g_stream));
+    return 0;
+}
+
+int lbm_batch_autotune(lbm_batch b, int trial_steps, int *block, int *grid, double *us_per_step) {
+    if (!b) return fail(-1, "null batch");
+    if (trial_steps < 1) trial_steps = 1;
+    // tuning advances the pool: save it and restore afterwards
+    void *save = nullptr;
+    HIP_TRY(hipMalloc(&save, b->pool_bytes));
+    int rc = lbm_field_copy_raw(save, b->pool, b->pool_bytes);
+    const int saved_parity = b->parity_stream, sb = b->block, sg = b->grid;
+    double best = 1e300;
+    int bb = 256, bg = 0;
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) == hipSuccess)
+        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    const int blocks[3] = {64, 128, 256};
+    for (int bi = 0; !rc && bi < 3; ++bi) {
+        const int per_cu = 2048 / blocks[bi];
+        const int grids[4] = {cus * per_cu / 4, cus * per_cu / 2, cus * per_cu, cus * per_cu * 4};
+        for (int gi = 0; !rc && gi < 4; ++gi) {
+            b->block = blocks[bi];
+            b->grid = grids[gi];
+            double ms = 0.0;
+            rc = lbm_batch_simulate(b, trial_steps + 1, 0, &ms);
+            const double us = 1e3 * ms / trial_steps;
+            if (!rc && us < best) {
+                best = us;
+                bb = blocks[bi];
+                bg = grids[gi];
+            }
+        }
+    }
+    if (!rc) rc = lbm_field_copy_raw(b->pool, save, b->pool_bytes);
+    (void)hipStreamSynchronize(g_stream);
+    (void)hipFree(save);
+    b->parity_stream = saved_parity;
+    batch_drop_graph(b);
+    if (rc) {
+        b->block = sb;
+        b->grid = sg;
+        return rc;
+    }
+    b->block = bb;
+    b->grid = bg;
+    if (block) *block = bb;
+    if (grid) *grid = bg;
+    if (us_per_step) *us_per_step = best;
     return 0;
 }
 

@@ -81,6 +81,22 @@ inline StepParams<float> to_float_params(const StepParams<double> &d) {
     return f;
 }
 
+// Batched lattices: one entry per lattice and one per tile of consecutive
+// nodes (row-major within the lattice).  Offsets are in elements of the pool.
+struct BatchLattice {
+    int nx, ny;
+    long long plane;
+    long long off[2];  // ping-pong buffers
+    double omo, om, oms, oma;  // SRT / TRT relaxation constants
+    int coll;                  // 0 SRT, 1 TRT
+    int pad;
+};
+
+struct BatchTile {
+    int lat;
+    int n0;
+};
+
 enum class Op : int { Collide = 0, StreamBC = 1, Fused = 2, ApplyBC = 3, Copy = 4, Convert = 5 };
 
 struct Launch {
@@ -107,6 +123,14 @@ int dispatch_lp(const Launch &L, const StepParams<double> &p, hipStream_t s);
     LBM_DECLARE_RUN(TAG, T, 4) LBM_DECLARE_RUN(TAG, T, 5) LBM_DECLARE_RUN(TAG, T, 9) \
     LBM_DECLARE_RUN(TAG, T, 13) LBM_DECLARE_RUN(TAG, T, 17) LBM_DECLARE_RUN(TAG, T, 21) \
     LBM_DECLARE_RUN(TAG, T, 37)
+// batched periodic D2Q9 step: op 0 collide, 1 fused step, 2 stream
+int run_batch_f64_q9(int op, const BatchLattice *lats, const BatchTile *tiles, int ntiles,
+                     void *pool, int parity, int block, int grid, const StepParams<double> &p,
+                     hipStream_t s);
+int run_batch_f32_q9(int op, const BatchLattice *lats, const BatchTile *tiles, int ntiles,
+                     void *pool, int parity, int block, int grid, const StepParams<float> &p,
+                     hipStream_t s);
+
 LBM_DECLARE_ALL_Q(f64, double)
 LBM_DECLARE_ALL_Q(f32, float)
 LBM_DECLARE_ALL_Q(f16dev, float)

@@ -921,6 +921,60 @@ int launch_op(const Launch &L, const StepParams<typename ST::T> &p, hipStream_t 
     return hipGetLastError() == hipSuccess ? 0 : -4;
 }
 
+// ---------------------------------------------------------------------------
+// batched independent periodic lattices: a persistent grid strides over a
+// table of tiles (blockDim.x consecutive nodes of one lattice each); lattice
+// geometry and relaxation times come from the lattice table (uniform per
+// tile, so they live in scalar registers).  OP: 0 collide, 1 fused step,
+// 2 stream.  All lattices share the ping-pong parity.
+// ---------------------------------------------------------------------------
+template <int QID, class ST, int OP>
+__global__ void __launch_bounds__(256)
+k_batch(const BatchLattice *__restrict__ lats, const BatchTile *__restrict__ tiles, int ntiles,
+        typename ST::store_t *__restrict__ pool, int parity,
+        const StepParams<typename ST::T> base) {
+    using T = typename ST::T;
+    using S = typename ST::store_t;
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const BatchTile tile = tiles[t];
+        const BatchLattice L = lats[tile.lat];
+        StepParams<T> p = base;
+        p.nx = L.nx;
+        p.ny = L.ny;
+        p.plane = L.plane;
+        p.omo = static_cast<T>(L.omo);
+        p.om = static_cast<T>(L.om);
+        p.oms = static_cast<T>(L.oms);
+        p.oma = static_cast<T>(L.oma);
+        const S *src = pool + L.off[parity];
+        S *dst = pool + L.off[parity ^ 1];
+        const int n = tile.n0 + (int)threadIdx.x;
+        if (n >= (int)L.plane) continue;
+        const int y = n / L.nx;
+        const int x = n - y * L.nx;
+        T f[QT<QID>::Q];
+        if constexpr (OP == 0) load_local<QID, ST>(src, p, n, f, LBM_SEQ);
+        else pull_seq<QID, ST>(src, p, x, y, f, LBM_SEQ);
+        if constexpr (OP != 2) {
+            if (L.coll == 0) collide_any<QID, 0>(f, p, x, y);
+            else collide_any<QID, 1>(f, p, x, y);
+        }
+        store_local<QID, ST>(dst, p, n, f, LBM_SEQ);
+    }
+}
+
+#define LBM_DEFINE_BATCH(TAG, ST, QID)                                                       \
+    int run_batch_##TAG##_q##QID(int op, const BatchLattice *lats, const BatchTile *tiles,   \
+                                 int ntiles, void *pool, int parity, int block, int grid,    \
+                                 const StepParams<typename ST::T> &p, hipStream_t s) {      \
+        using S = typename ST::store_t;                                                      \
+        S *pl = static_cast<S *>(pool);                                                      \
+        if (op == 0) k_batch<QID, ST, 0><<<grid, block, 0, s>>>(lats, tiles, ntiles, pl, parity, p); \
+        else if (op == 1) k_batch<QID, ST, 1><<<grid, block, 0, s>>>(lats, tiles, ntiles, pl, parity, p); \
+        else k_batch<QID, ST, 2><<<grid, block, 0, s>>>(lats, tiles, ntiles, pl, parity, p); \
+        return hipGetLastError() == hipSuccess ? 0 : -4;                                     \
+    }
+
 // One translation unit per (storage, quadrature group) instantiates these
 // entry points; lbm_dispatch.cpp routes a launch to them.
 #define LBM_DEFINE_RUN(TAG, ST, QID)                                                        \

@@ -31,6 +31,8 @@ EXPORTS = [
     "lbm_autotune",
     "lbm_collide", "lbm_stream", "lbm_apply_bcs", "lbm_stream_bc", "lbm_simulate",
     "lbm_simulate_timed", "lbm_moments", "lbm_reduce",
+    "lbm_batch_create", "lbm_batch_destroy", "lbm_batch_info", "lbm_batch_upload",
+    "lbm_batch_download", "lbm_batch_set_launch", "lbm_batch_autotune", "lbm_batch_simulate",
 ]
 
 
@@ -97,6 +99,14 @@ def lib() -> ctypes.CDLL:
     L.lbm_simulate_timed.argtypes = [vp, vp, vp, i64, dp, ctypes.POINTER(i64)]
     L.lbm_moments.argtypes = [vp, d, dp, dp, dp, dp, dp, dp]
     L.lbm_reduce.argtypes = [vp, dp]
+    L.lbm_batch_create.argtypes = [ctypes.POINTER(vp), i, i, i, ip, ip, dp, dp, d]
+    L.lbm_batch_destroy.argtypes = [vp]
+    L.lbm_batch_info.argtypes = [vp, ip, ctypes.POINTER(i64), ctypes.POINTER(i64)]
+    L.lbm_batch_upload.argtypes = [vp, i, dp]
+    L.lbm_batch_download.argtypes = [vp, i, i, dp]
+    L.lbm_batch_set_launch.argtypes = [vp, i, i]
+    L.lbm_batch_autotune.argtypes = [vp, i, ip, ip, dp]
+    L.lbm_batch_simulate.argtypes = [vp, i64, i, dp]
     for name in EXPORTS:
         f = getattr(L, name)
         if name not in ("lbm_last_error",):
