@@ -133,6 +133,56 @@ def batch_leg(L, steps=100, count=512, seed=0):
             "speedup_vs_sequential": round(seq / wall_b, 2)}
 
 
+def adjoint_leg(L, n=512, T=2000, tau=0.7):
+    """Reverse-mode gradients of L = 1/2 |u(T) - u_target|^2 for D2Q9 SRT on a
+    periodic n x n lattice: u0 and u_target are divergence-free fields from
+    Fourier modes |k| <= 8 at amplitude 0.02 (seeds 0 and 1), sqrt(T)
+    checkpointing.  FP64 gradients are checked against h = 1e-4 central
+    differences (tau, and one divergence-free direction in u0); FP32 storage
+    gradients against FP64.  One gradient = forward pass + recomputation +
+    reverse pass, 3 T lattice sweeps."""
+    q = L.D2Q9()
+    u0 = L.divergence_free_field((n, n), 8, 0.02, seed=0)
+    ut = L.divergence_free_field((n, n), 8, 0.02, seed=1)
+    out = {"workload": f"D2Q9 SRT tau={tau}, periodic {n}x{n}, T={T}, loss 1/2|u(T)-u_target|^2"}
+    res = {}
+    for st in ("f64", "f32"):
+        prob = L.AdjointProblem(q, L.SRT(tau), (n, n), T, storage=st)
+        prob.gradient(u0, ut)                                   # warm-up
+        L.synchronize()
+        t0 = time.perf_counter()
+        r = prob.gradient(u0, ut)
+        wall = time.perf_counter() - t0
+        res[st] = r
+        info = prob.info()
+        out[st] = {"gradient_ms": round(wall * 1e3, 2), "device_ms": round(r["ms"], 2),
+                   "checkpoint_every": info["checkpoint_every"],
+                   "device_bytes": info["bytes"], "loss": r["loss"],
+                   "grad_tau": r["grad_tau"],
+                   "sweeps_mlups": round(3 * n * n * T / wall / 1e6, 1)}
+        if st == "f64":
+            h = 1e-4
+            prob.set_collision(L.SRT(tau + h))
+            lp = prob.loss(u0, ut)
+            prob.set_collision(L.SRT(tau - h))
+            lm = prob.loss(u0, ut)
+            prob.set_collision(L.SRT(tau))
+            fd_t = (lp - lm) / (2 * h)
+            v = L.divergence_free_field((n, n), 8, 0.02, seed=2)
+            fd_u = (prob.loss(u0 + h * v, ut) - prob.loss(u0 - h * v, ut)) / (2 * h)
+            ad_u = float(np.sum(r["grad_u0"] * v))
+            out[st]["fd_rel_err_tau"] = abs(r["grad_tau"] - fd_t) / abs(fd_t)
+            out[st]["fd_rel_err_u0_direction"] = abs(ad_u - fd_u) / abs(fd_u)
+        prob.close()
+    a, b = res["f32"], res["f64"]
+    out["f32_vs_f64"] = {
+        "grad_u0_rel_l2": float(np.linalg.norm(a["grad_u0"] - b["grad_u0"])
+                                / np.linalg.norm(b["grad_u0"])),
+        "grad_tau_rel": abs(a["grad_tau"] / b["grad_tau"] - 1.0),
+        "loss_rel": abs(a["loss"] / b["loss"] - 1.0)}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -144,6 +194,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp16", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
@@ -241,6 +292,9 @@ def main():
     if not a.no_batch:
         print("[bench] batch of 512 lattices", file=sys.stderr, flush=True)
         result["batch"] = batch_leg(L)
+    if not a.no_adjoint:
+        print("[bench] adjoint gradients 512^2, T=2000", file=sys.stderr, flush=True)
+        result["adjoint"] = adjoint_leg(L)
     if not a.no_cpu:
         print("[bench] CPU baseline (oracle restatement)", file=sys.stderr, flush=True)
         result["cpu_baseline"] = cpu_baseline(n, cm.τ_symmetric, cm.τ_asymmetric, f0,

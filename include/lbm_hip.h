@@ -174,6 +174,33 @@ int lbm_batch_autotune(lbm_batch b, int trial_steps, int *block, int *grid, doub
  * NULL) receives the time of the fused steps (HIP events). */
 int lbm_batch_simulate(lbm_batch b, int64_t nsteps, int graph_chunk, double *ms_fused);
 
+/* ---- reverse mode (gradients) ------------------------------------------ */
+/* Differentiable run of a periodic, unforced SRT or TRT model:
+ *   S_0 = feq(rho0, u0) (the equilibrium initial condition,
+ *   initial_conditions.jl:7-22), T steps of collide! -> stream!
+ *   (lattice_boltzmann_model.jl:60-70), loss L = 1/2 sum_x |u(S_T) - u_target|^2.
+ * lbm_adjoint_gradient returns dL/du0 and dL/dtau by a hand-written reverse
+ * pass (transposed stream = push, transposed collision Jacobian per node),
+ * keeping every checkpoint_every-th post-collision field (0 = ceil(sqrt(T)))
+ * and recomputing the states in between.  No counterpart in the reference
+ * (it is differentiated only by running it); the model (collision and tau)
+ * is read at each call and must outlive the handle.  Storage LBM_STORE_F64
+ * or LBM_STORE_F32 (shifted populations, FP32 arithmetic).
+ * u0, u_target, u_final, grad_u0: (nx, ny, 2) column-major Float64 host arrays. */
+typedef struct lbm_adjoint_s *lbm_adjoint;
+int lbm_adjoint_create(lbm_adjoint *out, lbm_model m, int storage, int nx, int ny, double rho0,
+                       int64_t nsteps, int checkpoint_every);
+int lbm_adjoint_destroy(lbm_adjoint a);
+/* checkpoint interval, number of stored checkpoints, device bytes held */
+int lbm_adjoint_info(lbm_adjoint a, int *checkpoint_every, int *n_checkpoints, int64_t *bytes);
+/* forward only: loss and (u_final may be NULL) the velocity field u(S_T) */
+int lbm_adjoint_loss(lbm_adjoint a, const double *u0, const double *u_target, double *loss,
+                     double *u_final);
+/* loss, dL/du0 (may be NULL) and dL/dtau: grad_tau[0] = dL/dtau (SRT) or
+ * grad_tau[0..1] = dL/dtau_s, dL/dtau_a (TRT); *ms (may be NULL) = device time */
+int lbm_adjoint_gradient(lbm_adjoint a, const double *u0, const double *u_target, double *loss,
+                         double *grad_u0, double *grad_tau, double *ms);
+
 /* ---- moments ------------------------------------------------------------ */
 /* density / velocity! / deviatoric_tensor per node (moments.jl:1-19,199-214),
  * as TrackHydrodynamicErrors uses them (track_hydrodynamic_errors.jl:139-177).

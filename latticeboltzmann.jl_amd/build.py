@@ -5,7 +5,7 @@
 
 Translation units and their floating-point contracts:
 
-* ``k_f64_*.hip``, ``kernels_aux.hip``, ``lbm_capi.cpp``, ``lbm_dispatch.cpp``:
+* ``k_*f64*.hip``, ``kernels_aux.hip``, ``lbm_capi.cpp``, ``lbm_dispatch.cpp``:
   ``-ffp-contract=off`` -- every multiply and add rounds separately, the
   reference's Julia semantics, so the FP64 path reproduces the CPU restatement
   bit for bit;
@@ -49,12 +49,11 @@ def _units():
              ("lbm_dispatch.cpp", ["-x", "hip", "-ffp-contract=off"])]
     for name in sorted(os.listdir(CSRC)):
         if name.startswith("k_") and name.endswith(".hip"):
-            contract = "off" if (name.startswith("k_f64_") or name.startswith("k_batch_f64")) \
-                else "fast"
+            contract = "off" if "_f64" in name else "fast"
             units.append((name, [f"-ffp-contract={contract}"]))
     return units
 
-HEADERS = ["lbm_quad.h", "lbm_internal.h", "lbm_kernels.h"]
+HEADERS = ["lbm_quad.h", "lbm_internal.h", "lbm_kernels.h", "lbm_adjoint.h"]
 
 
 def _mtime(p):

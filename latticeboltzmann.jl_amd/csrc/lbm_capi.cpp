@@ -411,6 +411,7 @@ int lbm_field_create(lbm_field *out, int quadrature, int nx, int ny, int storage
     HostQuad q;
     if (!host_quad(quadrature, q)) return fail(-1, "unknown quadrature D2Q%d", quadrature);
     if (nx < 1 || ny < 1) return fail(-1, "lattice size %dx%d must be positive", nx, ny);
+    if (ny > 65535) return fail(-1, "lattice height %d exceeds 65535 (one grid row per y)", ny);
     const int eb = storage_bytes(storage);
     if (!eb) return fail(-1, "unknown storage %d", storage);
     if ((long long)nx * (long long)ny > (1LL << 31) - 1)
@@ -1289,6 +1290,318 @@ int lbm_batch_autotune(lbm_batch b, int trial_steps, int *block, int *grid, doub
     if (block) *block = bb;
     if (grid) *grid = bg;
     if (us_per_step) *us_per_step = best;
+    return 0;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// reverse mode: gradients of L = 1/2 sum |u(S_T) - u_target|^2 with respect to
+// u0 (S_0 = feq(rho0, u0)) and the relaxation time(s).
+//
+// Checkpoints.  The forward pass keeps, every K steps, the post-collision
+// field C_{aK-1} from which the pull kernel re-forms S_{aK} (segment a starts
+// there; segment 0 starts from S_0).  The reverse pass walks the segments
+// backwards: it recomputes C_{t0} .. C_{t1-2} of the segment into K - 1
+// buffers, then runs the reverse steps t1-1 .. t0, each reading S_t by a pull
+// from C_{t-1}.  Storage: ceil(T/K) + K + 2 population fields; work: two
+// forward passes and one reverse pass.  K = ceil(sqrt(T)) by default.
+// ---------------------------------------------------------------------------
+struct lbm_adjoint_s {
+    lbm_model m = nullptr;
+    lbm_field_s geo;  // geometry and storage of the populations (no data)
+    int64_t nsteps = 0;
+    int K = 1, nseg = 0;
+    size_t state_bytes = 0, lam_bytes = 0, total_bytes = 0;
+    void *S0 = nullptr;
+    void *pp[2] = {nullptr, nullptr};
+    std::vector<void *> ck;   // ck[a] = C_{aK-1}, a = 1 .. nseg-1
+    std::vector<void *> seg;  // recomputed C_{t0} .. C_{t1-2} of one segment
+    void *lam[2] = {nullptr, nullptr};
+    double *du0 = nullptr, *dut = nullptr, *dgrad = nullptr, *dlossn = nullptr, *dgtau = nullptr,
+           *duT = nullptr, *dpart = nullptr, *dout = nullptr;
+};
+
+namespace {
+
+constexpr int kAdjParts = 1024;
+
+void adjoint_free(lbm_adjoint a) {
+    free_dev(a->S0);
+    free_dev(a->pp[0]);
+    free_dev(a->pp[1]);
+    for (void *&p : a->ck) free_dev(p);
+    for (void *&p : a->seg) free_dev(p);
+    free_dev(a->lam[0]);
+    free_dev(a->lam[1]);
+    for (double **p : {&a->du0, &a->dut, &a->dgrad, &a->dlossn, &a->dgtau, &a->duT, &a->dpart,
+                       &a->dout}) {
+        void *v = *p;
+        free_dev(v);
+        *p = nullptr;
+    }
+}
+
+int adjoint_check_model(lbm_model m, int qid, const char *what) {
+    if (!m) return fail(-1, "%s: null model", what);
+    if (m->qid != qid)
+        return fail(-1, "%s: the model is D2Q%d but the adjoint was made for D2Q%d", what, m->qid, qid);
+    if (m->coll != LBM_SRT && m->coll != LBM_TRT)
+        return fail(-2, "%s: the adjoint supports SRT and TRT collisions", what);
+    if (m->force_kind != LBM_FORCE_NONE)
+        return fail(-2, "%s: the adjoint supports unforced flows only", what);
+    if (!m->bcs.empty())
+        return fail(-2, "%s: the adjoint supports periodic lattices only (no boundary conditions)",
+                    what);
+    return 0;
+}
+
+int adj_launch(lbm_adjoint a, const lbm::StepParams<double> &p, lbm::AdjLaunch L) {
+    L.qid = a->geo.qid;
+    L.coll = a->m->coll == LBM_SRT ? 0 : 1;
+    const int rc = a->geo.storage == LBM_STORE_F64
+                       ? lbm::run_adjoint_f64(L, p, g_stream)
+                       : lbm::run_adjoint_f32(L, lbm::to_float_params(p), g_stream);
+    if (rc) return fail(-4, "adjoint kernel launch failed (code %d): %s", rc,
+                        hipGetErrorString(hipGetLastError()));
+    return 0;
+}
+
+// forward step kernels on raw buffers of the adjoint's geometry
+int adj_forward_op(lbm_adjoint a, lbm::Op op, const void *src, void *dst) {
+    lbm::Launch L{};
+    L.op = op;
+    L.qid = a->geo.qid;
+    L.coll = a->m->coll;
+    L.storage = a->geo.storage;
+    L.src = src;
+    L.dst = dst;
+    if (op == lbm::Op::Fused) auto_launch(a->m, &a->geo, L.block, L.block_y, L.vec, L.nt);
+    const lbm::StepParams<double> p = make_params(a->m, &a->geo, false);
+    const int rc = a->geo.storage == LBM_STORE_F64 ? lbm::dispatch_f64(L, p, g_stream)
+                                                   : lbm::dispatch_lp(L, p, g_stream);
+    if (rc) return fail(-4, "kernel launch failed (code %d): %s", rc,
+                        hipGetErrorString(hipGetLastError()));
+    return 0;
+}
+
+// post-collision buffer of forward step t: checkpoints at t = aK - 1
+void *adj_cbuf(lbm_adjoint a, int64_t t) {
+    if ((t + 1) % a->K == 0 && t + 1 < a->nsteps) return a->ck[(size_t)((t + 1) / a->K)];
+    return a->pp[t & 1];
+}
+
+// S_0 from u0, the forward pass with checkpoints, lambda_T and the loss
+int adjoint_forward(lbm_adjoint a, const double *u0, const double *ut, bool want_uT) {
+    const size_t vb = (size_t)a->geo.nx * a->geo.ny * 2 * sizeof(double);
+    HIP_TRY(hipMemcpyAsync(a->du0, u0, vb, hipMemcpyHostToDevice, g_stream));
+    HIP_TRY(hipMemcpyAsync(a->dut, ut, vb, hipMemcpyHostToDevice, g_stream));
+    const lbm::StepParams<double> p = make_params(a->m, &a->geo, false);
+    lbm::AdjLaunch L{};
+    L.what = 0;
+    L.pops = a->S0;
+    L.u = a->du0;
+    if (int rc = adj_launch(a, p, L)) return rc;
+    const int64_t T = a->nsteps;
+    if (T > 0) {
+        if (int rc = adj_forward_op(a, lbm::Op::Collide, a->S0, adj_cbuf(a, 0))) return rc;
+        for (int64_t t = 1; t < T; ++t)
+            if (int rc = adj_forward_op(a, lbm::Op::Fused, adj_cbuf(a, t - 1), adj_cbuf(a, t)))
+                return rc;
+    }
+    L = lbm::AdjLaunch{};
+    L.what = 1;
+    L.direct = T == 0 ? 1 : 0;
+    L.state = T == 0 ? a->S0 : adj_cbuf(a, T - 1);
+    L.u = a->dut;
+    L.lam_out = a->lam[0];
+    L.out = a->dlossn;
+    L.aux = want_uT ? a->duT : nullptr;
+    if (int rc = adj_launch(a, p, L)) return rc;
+    if (lbm::adjoint_sum(a->dlossn, (long long)a->geo.nx * a->geo.ny, a->dpart, kAdjParts,
+                         a->dout, g_stream))
+        return fail(-4, "reduction kernel failed");
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lbm_adjoint_create(lbm_adjoint *out, lbm_model m, int storage, int nx, int ny, double rho0,
+                       int64_t nsteps, int checkpoint_every) {
+    if (!out) return fail(-1, "lbm_adjoint_create: null output handle");
+    *out = nullptr;
+    if (!m) return fail(-1, "lbm_adjoint_create: null model");
+    if (int rc = adjoint_check_model(m, m->qid, "lbm_adjoint_create")) return rc;
+    if (storage != LBM_STORE_F64 && storage != LBM_STORE_F32)
+        return fail(-2, "lbm_adjoint_create: storage must be LBM_STORE_F64 or LBM_STORE_F32");
+    if (nx < 1 || ny < 1) return fail(-1, "lattice size %dx%d must be positive", nx, ny);
+    if (ny > 65535) return fail(-1, "lattice height %d exceeds 65535", ny);
+    if ((long long)nx * (long long)ny > (1LL << 31) - 1)
+        return fail(-1, "lattice of %lld nodes exceeds 2^31-1", (long long)nx * ny);
+    if (nsteps < 0) return fail(-1, "negative step count");
+    if (checkpoint_every < 0) return fail(-1, "checkpoint interval must be >= 0");
+    if (!(rho0 > 0.0)) return fail(-1, "rho0 must be positive");
+    if (int rc = ensure_stream()) return rc;
+    lbm_adjoint a = new lbm_adjoint_s;
+    a->m = m;
+    a->geo.qid = m->qid;
+    a->geo.nx = nx;
+    a->geo.ny = ny;
+    a->geo.Q = m->q.Q;
+    a->geo.storage = storage;
+    a->geo.rho0 = rho0;
+    a->geo.dscale = 1.0;
+    a->nsteps = nsteps;
+    int K = checkpoint_every;
+    if (K == 0) {
+        K = 1;
+        while ((int64_t)K * K < nsteps) ++K;
+    }
+    a->K = K;
+    a->nseg = nsteps > 0 ? (int)((nsteps + K - 1) / K) : 0;
+    const size_t plane = (size_t)nx * (size_t)ny;
+    const size_t eb = storage == LBM_STORE_F64 ? 8 : 4;
+    a->state_bytes = plane * (size_t)m->q.Q * eb;
+    a->lam_bytes = a->state_bytes;
+    const int64_t seglen = nsteps < K ? nsteps : K;
+    a->ck.assign(a->nseg > 0 ? (size_t)a->nseg : 1, nullptr);
+    a->seg.assign(seglen > 1 ? (size_t)(seglen - 1) : 0, nullptr);
+    auto alloc = [&](void **p, size_t bytes) -> int {
+        const hipError_t e = hipMalloc(p, bytes);
+        if (e != hipSuccess) {
+            *p = nullptr;
+            return fail(-3, "hipMalloc(%zu) for the adjoint failed: %s", bytes, hipGetErrorString(e));
+        }
+        a->total_bytes += bytes;
+        return 0;
+    };
+    int rc = alloc(&a->S0, a->state_bytes);
+    for (int k = 0; !rc && k < 2; ++k) rc = alloc(&a->pp[k], a->state_bytes);
+    for (size_t k = 1; !rc && k < a->ck.size(); ++k) rc = alloc(&a->ck[k], a->state_bytes);
+    for (size_t k = 0; !rc && k < a->seg.size(); ++k) rc = alloc(&a->seg[k], a->state_bytes);
+    for (int k = 0; !rc && k < 2; ++k) rc = alloc(&a->lam[k], a->lam_bytes);
+    void *v = nullptr;
+    if (!rc && !(rc = alloc(&v, plane * 2 * sizeof(double)))) a->du0 = (double *)v;
+    if (!rc && !(rc = alloc(&v, plane * 2 * sizeof(double)))) a->dut = (double *)v;
+    if (!rc && !(rc = alloc(&v, plane * 2 * sizeof(double)))) a->dgrad = (double *)v;
+    if (!rc && !(rc = alloc(&v, plane * 2 * sizeof(double)))) a->dgtau = (double *)v;
+    if (!rc && !(rc = alloc(&v, plane * 2 * sizeof(double)))) a->duT = (double *)v;
+    if (!rc && !(rc = alloc(&v, plane * sizeof(double)))) a->dlossn = (double *)v;
+    if (!rc && !(rc = alloc(&v, kAdjParts * sizeof(double)))) a->dpart = (double *)v;
+    if (!rc && !(rc = alloc(&v, 4 * sizeof(double)))) a->dout = (double *)v;
+    if (rc) {
+        adjoint_free(a);
+        delete a;
+        return rc;
+    }
+    *out = a;
+    return 0;
+}
+
+int lbm_adjoint_destroy(lbm_adjoint a) {
+    if (!a) return 0;
+    (void)hipStreamSynchronize(g_stream);
+    adjoint_free(a);
+    delete a;
+    return 0;
+}
+
+int lbm_adjoint_info(lbm_adjoint a, int *checkpoint_every, int *n_checkpoints, int64_t *bytes) {
+    if (!a) return fail(-1, "null adjoint");
+    if (checkpoint_every) *checkpoint_every = a->K;
+    if (n_checkpoints) *n_checkpoints = a->nseg > 0 ? a->nseg - 1 : 0;
+    if (bytes) *bytes = (int64_t)a->total_bytes;
+    return 0;
+}
+
+int lbm_adjoint_loss(lbm_adjoint a, const double *u0, const double *u_target, double *loss,
+                     double *u_final) {
+    if (!a) return fail(-1, "null adjoint");
+    if (!u0 || !u_target || !loss) return fail(-1, "lbm_adjoint_loss: null argument");
+    if (int rc = adjoint_check_model(a->m, a->geo.qid, "lbm_adjoint_loss")) return rc;
+    if (int rc = adjoint_forward(a, u0, u_target, u_final != nullptr)) return rc;
+    HIP_TRY(hipMemcpyAsync(loss, a->dout, sizeof(double), hipMemcpyDeviceToHost, g_stream));
+    if (u_final)
+        HIP_TRY(hipMemcpyAsync(u_final, a->duT, (size_t)a->geo.nx * a->geo.ny * 2 * sizeof(double),
+                               hipMemcpyDeviceToHost, g_stream));
+    HIP_TRY(hipStreamSynchronize(g_stream));
+    return 0;
+}
+
+int lbm_adjoint_gradient(lbm_adjoint a, const double *u0, const double *u_target, double *loss,
+                         double *grad_u0, double *grad_tau, double *ms) {
+    if (!a) return fail(-1, "null adjoint");
+    if (!u0 || !u_target) return fail(-1, "lbm_adjoint_gradient: null argument");
+    if (int rc = adjoint_check_model(a->m, a->geo.qid, "lbm_adjoint_gradient")) return rc;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (ms) {
+        HIP_TRY(hipEventCreate(&e0));
+        HIP_TRY(hipEventCreate(&e1));
+        HIP_TRY(hipEventRecord(e0, g_stream));
+    }
+    if (int rc = adjoint_forward(a, u0, u_target, false)) return rc;
+    const lbm::StepParams<double> p = make_params(a->m, &a->geo, false);
+    const long long plane = (long long)a->geo.nx * a->geo.ny;
+    HIP_TRY(hipMemsetAsync(a->dgtau, 0, (size_t)plane * 2 * sizeof(double), g_stream));
+    int cur = 0;  // lam[cur] holds lambda_{t+1}
+    for (int s = a->nseg - 1; s >= 0; --s) {
+        const int64_t t0 = (int64_t)s * a->K;
+        const int64_t t1 = t0 + a->K < a->nsteps ? t0 + a->K : a->nsteps;
+        // recompute C_{t0} .. C_{t1-2}
+        for (int64_t j = 0; t0 + j <= t1 - 2; ++j) {
+            int rc;
+            if (j > 0) rc = adj_forward_op(a, lbm::Op::Fused, a->seg[j - 1], a->seg[j]);
+            else if (s == 0) rc = adj_forward_op(a, lbm::Op::Collide, a->S0, a->seg[0]);
+            else rc = adj_forward_op(a, lbm::Op::Fused, a->ck[s], a->seg[0]);
+            if (rc) return rc;
+        }
+        for (int64_t t = t1 - 1; t >= t0; --t) {
+            lbm::AdjLaunch L{};
+            L.what = 2;
+            if (t == t0) {
+                L.direct = s == 0 ? 1 : 0;
+                L.state = s == 0 ? a->S0 : a->ck[s];
+            } else {
+                L.direct = 0;
+                L.state = a->seg[t - 1 - t0];
+            }
+            L.lam_in = a->lam[cur];
+            L.lam_out = a->lam[cur ^ 1];
+            L.out = a->dgtau;
+            if (int rc = adj_launch(a, p, L)) return rc;
+            cur ^= 1;
+        }
+    }
+    lbm::AdjLaunch L{};
+    L.what = 3;
+    L.u = a->du0;
+    L.lam_in = a->lam[cur];
+    L.out = a->dgrad;
+    if (int rc = adj_launch(a, p, L)) return rc;
+    const int ntau = a->m->coll == LBM_SRT ? 1 : 2;
+    for (int k = 0; k < ntau; ++k)
+        if (lbm::adjoint_sum(a->dgtau + k * plane, plane, a->dpart, kAdjParts, a->dout + 1 + k,
+                             g_stream))
+            return fail(-4, "reduction kernel failed");
+    if (ms) HIP_TRY(hipEventRecord(e1, g_stream));
+    double h[3] = {0.0, 0.0, 0.0};
+    HIP_TRY(hipMemcpyAsync(h, a->dout, 3 * sizeof(double), hipMemcpyDeviceToHost, g_stream));
+    if (grad_u0)
+        HIP_TRY(hipMemcpyAsync(grad_u0, a->dgrad, (size_t)plane * 2 * sizeof(double),
+                               hipMemcpyDeviceToHost, g_stream));
+    HIP_TRY(hipStreamSynchronize(g_stream));
+    if (loss) *loss = h[0];
+    if (grad_tau)
+        for (int k = 0; k < ntau; ++k) grad_tau[k] = h[1 + k];
+    if (ms) {
+        float t = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&t, e0, e1));
+        *ms = (double)t;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+    }
     return 0;
 }
 

@@ -112,6 +112,25 @@ struct Launch {
     int nt = 0;   // fused step: non-temporal stores
 };
 
+// Reverse-mode kernels of the periodic SRT/TRT step (lbm_adjoint.h).
+struct AdjLaunch {
+    int what;            // 0 S0 = feq(rho0, u0), 1 seed lambda_T + loss, 2 reverse step, 3 dL/du0
+    int qid, coll;       // coll 0 SRT, 1 TRT
+    int direct;          // state stored directly (1) or pulled from a post-collision field (0)
+    const void *state;   // populations read by the seed / reverse step
+    void *pops;          // populations written by the initialisation
+    const void *lam_in;  // lambda_{t+1} (reverse step) or lambda_0 (dL/du0)
+    void *lam_out;       // lambda_t / lambda_T
+    const double *u;     // u0 (init, dL/du0) or u_target (seed), (nx, ny, 2) column-major
+    double *out;         // per-node loss (seed), dL/dtau accumulators (step), dL/du0
+    double *aux;         // seed: u(S_T), (nx, ny, 2) column-major, or nullptr
+};
+int run_adjoint_f64(const AdjLaunch &L, const StepParams<double> &p, hipStream_t s);
+int run_adjoint_f32(const AdjLaunch &L, const StepParams<float> &p, hipStream_t s);
+// deterministic sum of n doubles (fixed order for fixed n and npart)
+int adjoint_sum(const double *x, long long n, double *partials, int npart, double *out,
+                hipStream_t s);
+
 // Kernel dispatchers (lbm_dispatch.cpp): FP64 exact / reduced precision.
 int dispatch_f64(const Launch &L, const StepParams<double> &p, hipStream_t s);
 int dispatch_lp(const Launch &L, const StepParams<double> &p, hipStream_t s);

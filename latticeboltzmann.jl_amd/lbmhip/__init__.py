@@ -7,6 +7,7 @@ liblbm_hip.so (include/lbm_hip.h).  Julia names map as follows: ``collide!``
 :class:`HipLattice` (device memory, the reference's (nx, ny, Q) layout).
 """
 from ._lib import LbmError, lib  # noqa: F401
+from .adjoint import AdjointProblem, divergence_free_field, rrule, simulate_loss  # noqa: F401
 from .batch import LatticeBatch  # noqa: F401
 from .boundary_conditions import BounceBack, East, MovingWall, North, South, West  # noqa: F401
 from .collision_models import (MRT, SRT, TRT, TRT_Λ, collision_model_from_viscosity,  # noqa: F401

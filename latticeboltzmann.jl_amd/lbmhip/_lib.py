@@ -33,6 +33,8 @@ EXPORTS = [
     "lbm_simulate_timed", "lbm_moments", "lbm_reduce",
     "lbm_batch_create", "lbm_batch_destroy", "lbm_batch_info", "lbm_batch_upload",
     "lbm_batch_download", "lbm_batch_set_launch", "lbm_batch_autotune", "lbm_batch_simulate",
+    "lbm_adjoint_create", "lbm_adjoint_destroy", "lbm_adjoint_info", "lbm_adjoint_loss",
+    "lbm_adjoint_gradient",
 ]
 
 
@@ -107,6 +109,11 @@ def lib() -> ctypes.CDLL:
     L.lbm_batch_set_launch.argtypes = [vp, i, i]
     L.lbm_batch_autotune.argtypes = [vp, i, ip, ip, dp]
     L.lbm_batch_simulate.argtypes = [vp, i64, i, dp]
+    L.lbm_adjoint_create.argtypes = [ctypes.POINTER(vp), vp, i, i, i, d, i64, i]
+    L.lbm_adjoint_destroy.argtypes = [vp]
+    L.lbm_adjoint_info.argtypes = [vp, ip, ip, ctypes.POINTER(i64)]
+    L.lbm_adjoint_loss.argtypes = [vp, dp, dp, dp, dp]
+    L.lbm_adjoint_gradient.argtypes = [vp, dp, dp, dp, dp, dp, dp]
     for name in EXPORTS:
         f = getattr(L, name)
         if name not in ("lbm_last_error",):

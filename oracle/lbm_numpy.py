@@ -163,7 +163,7 @@ def equilibrium_field(q: Quadrature, rho, ux, uy) -> np.ndarray:
     s = q.s
     rho = np.asarray(rho, dtype=np.float64)
     usq = ux * ux + uy * uy
-    out = np.empty(rho.shape + (q.Q,))
+    out = np.empty(rho.shape + (q.Q,), dtype=np.result_type(rho, ux, uy, 1.0))
     for i in range(q.Q):
         ud = q.c[0, i] * ux + q.c[1, i] * uy
         x = s * ud
