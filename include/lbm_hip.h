@@ -49,10 +49,10 @@ enum { LBM_STORE_F64 = 0, LBM_STORE_F32 = 1, LBM_STORE_F16DEV = 2, LBM_STORE_BF1
 enum { LBM_SRT = 0, LBM_TRT = 1, LBM_MRT = 2 };
 
 /* body force, evaluated on the host (the reference's per-node closure
- * srt.jl:71, trt.jl:158, mrt.jl:92 cannot cross the ABI) */
+ * srt.jl:52, trt.jl:77, mrt.jl:92 cannot cross the ABI) */
 enum { LBM_FORCE_NONE = 0, LBM_FORCE_CONST = 1, LBM_FORCE_FIELD = 2 };
 
-/* boundary conditions: src/boundary_conditions.jl:102-106 */
+/* boundary conditions: src/boundary_conditions.jl:28-32, bounce_back.jl:2-6, moving_wall.jl:5-15 */
 enum { LBM_BOUNCE_BACK = 0, LBM_MOVING_WALL = 1 };
 enum { LBM_NORTH = 0, LBM_EAST = 1, LBM_SOUTH = 2, LBM_WEST = 3 };
 
@@ -61,7 +61,7 @@ typedef struct {
     int32_t direction;  /* LBM_NORTH .. LBM_WEST (moving wall: North only) */
     int32_t x0, x1;     /* bc.xs as a 1-based inclusive range */
     int32_t y0, y1;     /* bc.ys as a 1-based inclusive range */
-    double ux, uy;      /* MovingWall u (moving_wall.jl:186-196) */
+    double ux, uy;      /* MovingWall u (moving_wall.jl:10-11) */
     double rho;         /* MovingWall rho */
 } lbm_bc;
 
@@ -98,9 +98,9 @@ int lbm_field_copy(lbm_field dst, lbm_field src);
 /* ---- model (collision model, force, boundary conditions) --------------- */
 int lbm_model_create(lbm_model *out, int quadrature);
 int lbm_model_destroy(lbm_model m);
-/* SRT(tau) -- srt.jl:20-24 */
+/* SRT(tau) -- srt.jl:1-5 */
 int lbm_model_set_srt(lbm_model m, double tau);
-/* TRT(tau_s, tau_a, force) field order -- trt.jl:82-87 (note the reference's
+/* TRT(tau_s, tau_a, force) field order -- trt.jl:1-6 (note the reference's
  * 2-argument TRT(a, b) means tau_a = a, tau_s = b; the binding must swap) */
 int lbm_model_set_trt(lbm_model m, double tau_symmetric, double tau_asymmetric);
 /* MRT(q, taus) -- mrt.jl:19-34; taus[n-1] relaxes Hermite order n */
@@ -108,7 +108,7 @@ int lbm_model_set_mrt(lbm_model m, const double *taus, int ntaus);
 /* LBM_FORCE_NONE; LBM_FORCE_CONST: values = {Fx, Fy}; LBM_FORCE_FIELD:
  * values = (nx, ny, 2) column-major lattice force, copied to the device. */
 int lbm_model_set_force(lbm_model m, int kind, const double *values, int nx, int ny);
-/* ordered list, applied in order (boundary_conditions.jl:80-90) */
+/* ordered list, applied in order (boundary_conditions.jl:6-16) */
 int lbm_model_set_bcs(lbm_model m, const lbm_bc *bcs, int n);
 
 /* Launch shape of the fused step kernel: threads per block along x (64, 128,
@@ -126,13 +126,13 @@ int lbm_autotune(lbm_model m, lbm_field f_stream, int trial_steps, int *block_x,
                  double *us_per_step);
 
 /* ---- step operators ----------------------------------------------------- */
-/* collide!(cm, q, f_in, f_out; time) -- collision_models.jl:19, srt.jl:37-81,
- * trt.jl:123-178, mrt.jl:49-118.  f_in and f_out must be distinct. */
+/* collide!(cm, q, f_in, f_out; time) -- collision_models.jl:19, srt.jl:18-62,
+ * trt.jl:42-97, mrt.jl:49-118.  f_in and f_out must be distinct. */
 int lbm_collide(lbm_model m, lbm_field f_in, lbm_field f_out);
 /* stream!(q, f, f_new) -- stream.jl:18-30 (periodic pull, mod1 wrap) */
 int lbm_stream(lbm_model m, lbm_field f, lbm_field f_new);
-/* apply!(bcs, q, f_new, f_old; time) -- boundary_conditions.jl:80-94,
- * bounce_back.jl:117-181, moving_wall.jl:198-219.  Overwrites f_new in place. */
+/* apply!(bcs, q, f_new, f_old; time) -- boundary_conditions.jl:6-20,
+ * bounce_back.jl:8-72, moving_wall.jl:17-38.  Overwrites f_new in place. */
 int lbm_apply_bcs(lbm_model m, lbm_field f_new, lbm_field f_old);
 /* stream! then apply!, as one pass */
 int lbm_stream_bc(lbm_model m, lbm_field f, lbm_field f_new);
@@ -202,12 +202,12 @@ int lbm_adjoint_gradient(lbm_adjoint a, const double *u0, const double *u_target
                          double *grad_u0, double *grad_tau, double *ms);
 
 /* ---- moments ------------------------------------------------------------ */
-/* density / velocity! / deviatoric_tensor per node (moments.jl:1-19,199-214),
+/* density / velocity! / deviatoric_tensor per node (moments.jl:1-19,81-96),
  * as TrackHydrodynamicErrors uses them (track_hydrodynamic_errors.jl:139-177).
  * Outputs are (nx, ny) column-major Float64 host arrays; any may be NULL. */
 int lbm_moments(lbm_field f, double tau, double *rho, double *ux, double *uy,
                 double *sxx, double *sxy, double *syy);
-/* FP64 global sums over the lattice (track_hydrodynamic_errors.jl:192-202):
+/* FP64 global sums over the lattice (track_hydrodynamic_errors.jl:201-203):
  * out[0] = sum rho, out[1] = sum rho ux, out[2] = sum rho uy,
  * out[3] = sum rho |u|^2, out[4] = number of nodes with a non-finite population */
 int lbm_reduce(lbm_field f, double *out5);

@@ -67,7 +67,7 @@ k_convert(const double *__restrict__ hostlike, void *dev, int Q, int to_device,
 }
 
 // density / velocity! / deviatoric_tensor per node, in the CPU restatement's order
-// (moments.jl:1-19, 199-214; hermite_polynomials.jl:124-128; hermite.jl:45-48)
+// (moments.jl:1-19, 81-96; hermite_polynomials.jl:47-51; hermite.jl:45-48)
 template <int QID, int SID>
 __global__ void __launch_bounds__(256)
 k_moments(const void *f, const StepParams<double> p, double tau, double *__restrict__ rho_o,

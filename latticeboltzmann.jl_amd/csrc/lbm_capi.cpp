@@ -162,7 +162,7 @@ int storage_bytes(int storage) {
     }
 }
 
-// Hermite tensor H_n(c) scaled by cs = 1/s (src/hermite_polynomials.jl:124-159),
+// Hermite tensor H_n(c) scaled by cs = 1/s (src/hermite_polynomials.jl:47-82),
 // flattened column-major.
 int delta(int a, int b) { return a == b ? 1 : 0; }
 void hermite_tensor(int n, int cx, int cy, double cs, double *H) {
@@ -656,10 +656,10 @@ int lbm_model_set_bcs(lbm_model m, const lbm_bc *bcs, int n) {
         if (c.direction < LBM_NORTH || c.direction > LBM_WEST)
             return fail(-1, "unknown boundary direction %d", c.direction);
         if (c.kind == LBM_MOVING_WALL && c.direction != LBM_NORTH)
-            return fail(-2, "MovingWall is defined for North only (moving_wall.jl:198)");
+            return fail(-2, "MovingWall is defined for North only (moving_wall.jl:17)");
         dev[b] = lbm::BCDev{c.kind, c.direction, c.x0 - 1, c.x1 - 1, c.y0 - 1, c.y1 - 1};
         if (c.kind == LBM_MOVING_WALL) {
-            // a_1 = w_i * s * dot(rho_w u_w, H1(c_i)); f += 2 a_1 (moving_wall.jl:201-214)
+            // a_1 = w_i * s * dot(rho_w u_w, H1(c_i)); f += 2 a_1 (moving_wall.jl:20-24, 33)
             const double a1eq[2] = {c.rho * c.ux, c.rho * c.uy};
             for (int i = 0; i < m->q.Q; ++i) {
                 const double H1[2] = {(double)m->q.cx[i] * 1.0, (double)m->q.cy[i] * 1.0};

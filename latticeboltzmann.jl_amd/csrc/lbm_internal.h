@@ -24,7 +24,7 @@ struct StepParams {
     long long plane;  // nx * ny
     T w[37];
     // equilibrium constants (reference: maxwell_boltzmann_equilibrium.jl:43-66,
-    // velocity_distribution_function/quadratures.jl:69-190)
+    // velocity_distribution_function/quadratures.jl:3-159)
     T s, neg_s, s2, three_s, six_s3, three_s2, s4;
     // SRT: omo = 1 - 1/tau, om = 1/tau; TRT: oms = 1/tau_s, oma = 1/tau_a
     T omo, om, oms, oma;

@@ -183,7 +183,7 @@ __device__ __forceinline__ T udot(T ux, T uy) {
 }
 
 // Closed-form equilibrium of population I (maxwell_boltzmann_equilibrium.jl:43-66;
-// velocity_distribution_function/quadratures.jl:69-85, 111-127, 161-225).
+// velocity_distribution_function/quadratures.jl:3-19, 45-61, 95-124, 129-159).
 template <int QID, int I, class T>
 __device__ __forceinline__ T feq_i(T rho, T ux, T uy, T usq, const StepParams<T> &p) {
     constexpr int FEQ = QT<QID>::FEQ;
@@ -369,7 +369,7 @@ __device__ __forceinline__ void collide_seq(T (&f)[QT<QID>::Q], const StepParams
     if (p.force_kind != 0) {
         T Fx, Fy;
         node_force(p, x, y, Fx, Fy);
-        ux = ux + p.tau_force * Fx;  // srt.jl:73 / trt.jl:160
+        ux = ux + p.tau_force * Fx;  // srt.jl:54 / trt.jl:79
         uy = uy + p.tau_force * Fy;
     }
     T feq[Q];
@@ -386,10 +386,10 @@ __device__ __forceinline__ void collide_seq(T (&f)[QT<QID>::Q], const StepParams
         ((feq[Is] = feq_i<QID, Is, T>(rho, ux, uy, usq, p)), ...);
     }
     if constexpr (COLL == 0) {
-        // srt.jl:76-78
+        // srt.jl:57-59
         ((f[Is] = p.omo * f[Is] + p.om * feq[Is]), ...);
     } else {
-        // trt.jl:163-175 (each population reads the pre-collision pair)
+        // trt.jl:82-94 (each population reads the pre-collision pair)
         T g[Q];
         ((g[Is] = f[Is]), ...);
         auto trt = [&](auto Ic) {
@@ -470,7 +470,7 @@ __device__ __forceinline__ void collide_mrt_node(T (&f)[QT<QID>::Q], const StepP
 
 // ---------------------------------------------------------------------------
 // boundary conditions of the ordered list for node (x, y), applied to the
-// pulled populations f (bounce_back.jl:117-181, moving_wall.jl:198-219):
+// pulled populations f (bounce_back.jl:8-72, moving_wall.jl:17-38):
 // population i is replaced by the same node's post-collision opp(i) (+ the
 // moving-wall term) when its pull source lies beyond the wall.
 // ---------------------------------------------------------------------------

@@ -1,0 +1,295 @@
+"""
+    LBMHIP
+
+Julia binding of liblbm_hip.so (include/lbm_hip.h): the collide-stream step of
+LatticeBoltzmann.jl on one MI355X.  `HipLattice` is a device population array
+of the reference's layout (nx, ny, Q); methods of the reference's own
+`collide!`, `stream!`, `apply!` and `simulate` for it call the C ABI with
+`ccall`, so reference code runs unchanged once its arrays are `HipLattice`s.
+
+    using LatticeBoltzmann, LBMHIP
+    f = HipLattice(q, Array(f_host))               # upload
+    collide!(SRT(0.8), q, f, g)                    # srt.jl:18-24
+    stream!(q, g, f)                               # stream.jl:18-19
+    Array(f)                                       # download
+"""
+module LBMHIP
+
+using LatticeBoltzmann
+using LatticeBoltzmann: Quadrature, CollisionModel, SRT, TRT, MRT, BoundaryCondition,
+    BounceBack, MovingWall, North, East, South, West, LatticeBoltzmannModel
+import LatticeBoltzmann: collide!, stream!, apply!, simulate
+import ChainRulesCore
+
+export HipLattice, simulate_loss, adjoint_gradient
+
+const libpath = get(ENV, "LBMHIP_LIB", joinpath(@__DIR__, "..", "lib", "liblbm_hip.so"))
+
+const STORAGE = Dict(:f64 => 0, :f32 => 1, :f16dev => 2, :bf16dev => 3, :f16neq => 4,
+                     :bf16neq => 5)
+const FORCE_NONE, FORCE_CONST, FORCE_FIELD = 0, 1, 2
+
+function check(rc::Cint, what)
+    rc == 0 && return nothing
+    msg = unsafe_string(ccall((:lbm_last_error, libpath), Cstring, ()))
+    rc == -1 && throw(ArgumentError("$what: $msg"))
+    throw(ErrorException("$what: $msg"))
+end
+
+quadrature_id(q::Quadrature) = length(q.weights) == 25 ? 21 : length(q.weights)
+
+# ---- device population arrays ----------------------------------------------
+mutable struct HipLattice{Q <: Quadrature} <: AbstractArray{Float64, 3}
+    handle::Ptr{Cvoid}
+    q::Q
+    dims::NTuple{3, Int}
+    storage::Symbol
+    function HipLattice(q::Q, nx::Integer, ny::Integer; storage::Symbol = :f64,
+                        rho0::Real = 1.0) where {Q <: Quadrature}
+        h = Ref{Ptr{Cvoid}}(C_NULL)
+        check(ccall((:lbm_field_create, libpath), Cint,
+                    (Ptr{Ptr{Cvoid}}, Cint, Cint, Cint, Cint, Cdouble),
+                    h, quadrature_id(q), nx, ny, STORAGE[storage], rho0), "lbm_field_create")
+        a = new{Q}(h[], q, (Int(nx), Int(ny), length(q.weights)), storage)
+        finalizer(a) do x
+            x.handle == C_NULL || ccall((:lbm_field_destroy, libpath), Cint, (Ptr{Cvoid},),
+                                        x.handle)
+            x.handle = C_NULL
+        end
+    end
+end
+
+function HipLattice(q::Quadrature, f::Array{Float64, 3}; kw...)
+    a = HipLattice(q, size(f, 1), size(f, 2); kw...)
+    copyto!(a, f)
+end
+
+Base.size(a::HipLattice) = a.dims
+Base.IndexStyle(::Type{<:HipLattice}) = IndexLinear()
+Base.getindex(::HipLattice, ::Int) =
+    error("HipLattice lives in GPU memory: use Array(f) to read populations")
+Base.similar(a::HipLattice) = HipLattice(a.q, a.dims[1], a.dims[2]; storage = a.storage)
+Base.copy(a::HipLattice) = copyto!(similar(a), a)
+
+function Base.copyto!(a::HipLattice, f::Array{Float64, 3})
+    size(f) == a.dims || throw(DimensionMismatch("$(size(f)) vs $(a.dims)"))
+    GC.@preserve f check(ccall((:lbm_field_upload, libpath), Cint, (Ptr{Cvoid}, Ptr{Cdouble}),
+                               a.handle, f), "lbm_field_upload")
+    a
+end
+
+function Base.copyto!(f::Array{Float64, 3}, a::HipLattice)
+    size(f) == a.dims || throw(DimensionMismatch("$(size(f)) vs $(a.dims)"))
+    GC.@preserve f check(ccall((:lbm_field_download, libpath), Cint,
+                               (Ptr{Cvoid}, Ptr{Cdouble}), a.handle, f), "lbm_field_download")
+    f
+end
+
+function Base.copyto!(dst::HipLattice, src::HipLattice)
+    check(ccall((:lbm_field_copy, libpath), Cint, (Ptr{Cvoid}, Ptr{Cvoid}), dst.handle,
+                src.handle), "lbm_field_copy")
+    dst
+end
+
+Base.Array(a::HipLattice) = copyto!(Array{Float64, 3}(undef, a.dims), a)
+
+# ---- model handles: one per quadrature and thread ----------------------------
+mutable struct NativeModel
+    handle::Ptr{Cvoid}
+    key::Any
+end
+
+const MODELS = Dict{Tuple{Int, Int}, NativeModel}()
+
+function native_model(q::Quadrature)
+    get!(MODELS, (quadrature_id(q), Threads.threadid())) do
+        h = Ref{Ptr{Cvoid}}(C_NULL)
+        check(ccall((:lbm_model_create, libpath), Cint, (Ptr{Ptr{Cvoid}}, Cint), h,
+                    quadrature_id(q)), "lbm_model_create")
+        NativeModel(h[], nothing)
+    end
+end
+
+set_collision!(m::NativeModel, cm::SRT) =
+    check(ccall((:lbm_model_set_srt, libpath), Cint, (Ptr{Cvoid}, Cdouble), m.handle, cm.τ),
+          "lbm_model_set_srt")
+set_collision!(m::NativeModel, cm::TRT) =
+    check(ccall((:lbm_model_set_trt, libpath), Cint, (Ptr{Cvoid}, Cdouble, Cdouble), m.handle,
+                cm.τ_symmetric, cm.τ_asymmetric), "lbm_model_set_trt")
+function set_collision!(m::NativeModel, cm::MRT)
+    τs = Vector{Float64}(cm.τs)
+    GC.@preserve τs check(ccall((:lbm_model_set_mrt, libpath), Cint,
+                                (Ptr{Cvoid}, Ptr{Cdouble}, Cint), m.handle, τs, length(τs)),
+                          "lbm_model_set_mrt")
+end
+
+# The reference's force is a per-node closure (x_idx, y_idx, t) -> F; it is
+# evaluated here on the host and handed over as a FIELD buffer.
+function set_force!(m::NativeModel, cm::CollisionModel, nx, ny, time)
+    force = cm.force
+    if force === nothing
+        return check(ccall((:lbm_model_set_force, libpath), Cint,
+                           (Ptr{Cvoid}, Cint, Ptr{Cdouble}, Cint, Cint),
+                           m.handle, FORCE_NONE, C_NULL, 0, 0), "lbm_model_set_force")
+    end
+    F = Array{Float64, 3}(undef, nx, ny, 2)
+    for y in 1:ny, x in 1:nx
+        v = force(x, y, time)
+        F[x, y, 1], F[x, y, 2] = v[1], v[2]
+    end
+    GC.@preserve F check(ccall((:lbm_model_set_force, libpath), Cint,
+                               (Ptr{Cvoid}, Cint, Ptr{Cdouble}, Cint, Cint),
+                               m.handle, FORCE_FIELD, F, nx, ny), "lbm_model_set_force")
+end
+
+# lbm_bc (include/lbm_hip.h)
+struct CBC
+    kind::Int32
+    direction::Int32
+    x0::Int32
+    x1::Int32
+    y0::Int32
+    y1::Int32
+    ux::Cdouble
+    uy::Cdouble
+    rho::Cdouble
+end
+
+direction_id(::North) = 0
+direction_id(::East) = 1
+direction_id(::South) = 2
+direction_id(::West) = 3
+
+cbc(bc::BounceBack) = CBC(0, direction_id(bc.direction), first(bc.xs), last(bc.xs),
+                          first(bc.ys), last(bc.ys), 0.0, 0.0, 1.0)
+cbc(bc::MovingWall) = CBC(1, direction_id(bc.direction), first(bc.xs), last(bc.xs),
+                          first(bc.ys), last(bc.ys), bc.u[1], bc.u[2], bc.ρ)
+
+function set_bcs!(m::NativeModel, bcs)
+    list = CBC[cbc(bc) for bc in bcs]
+    GC.@preserve list check(ccall((:lbm_model_set_bcs, libpath), Cint,
+                                  (Ptr{Cvoid}, Ptr{CBC}, Cint), m.handle, list, length(list)),
+                            "lbm_model_set_bcs")
+end
+
+# ---- the reference's operators, for device arrays ---------------------------
+function collide!(cm::CollisionModel, q::Quadrature, f_in::HipLattice, f_out::HipLattice;
+                  time = 0.0)
+    m = native_model(q)
+    set_collision!(m, cm)
+    set_force!(m, cm, f_in.dims[1], f_in.dims[2], time)
+    check(ccall((:lbm_collide, libpath), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}),
+                m.handle, f_in.handle, f_out.handle), "lbm_collide")
+    nothing
+end
+
+collide!(cm::CollisionModel, q::Quadrature; time = 0.0, f_new::HipLattice, f_old::HipLattice) =
+    collide!(cm, q, f_old, f_new; time = time)
+
+function stream!(q::Quadrature, f::HipLattice, f_new::HipLattice)
+    check(ccall((:lbm_stream, libpath), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}),
+                native_model(q).handle, f.handle, f_new.handle), "lbm_stream")
+    nothing
+end
+
+stream!(q::Quadrature; f_new::HipLattice, f_old::HipLattice) = stream!(q, f_old, f_new)
+
+function apply!(bcs::AbstractVector{<:BoundaryCondition}, q::Quadrature, f_new::HipLattice,
+                f_old::HipLattice; time = 0.0)
+    m = native_model(q)
+    set_bcs!(m, bcs)
+    check(ccall((:lbm_apply_bcs, libpath), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}),
+                m.handle, f_new.handle, f_old.handle), "lbm_apply_bcs")
+    nothing
+end
+
+# simulate(model, time) (lattice_boltzmann_model.jl:60-77) with device arrays:
+# runs of steps between processing points go to lbm_simulate (one fused
+# collide-stream kernel per step); next! receives a host copy of f_stream at
+# the steps where the processing method does work (every step when it
+# processes, otherwise every 100th step and the last one).
+function simulate(model::LatticeBoltzmannModel{<:Quadrature, <:HipLattice}, time)
+    pm = model.processing_method
+    every = hasproperty(pm, :should_process) && pm.should_process ? 1 : 100
+    m = native_model(model.quadrature)
+    cm = model.collision_model
+    set_collision!(m, cm)
+    nx, ny, _ = size(model.f_stream)
+    set_bcs!(m, model.boundary_conditions)
+    t = first(time)
+    while t <= last(time)
+        stop = min(last(time), (div(t + 1, every) + 1) * every - 1)
+        n = stop - t + 1
+        set_force!(m, cm, nx, ny, t)  # constant-in-time forces only across a run
+        check(ccall((:lbm_simulate, libpath), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Int64),
+                    m.handle, model.f_stream.handle, model.f_collision.handle, n),
+              "lbm_simulate")
+        t = stop + 1
+        if LatticeBoltzmann.next!(pm, model.quadrature, Array(model.f_stream), t)
+            return model
+        end
+    end
+    model
+end
+
+# ---- reverse mode -----------------------------------------------------------
+# L(u0, τ) = 1/2 Σ |u(S_T) - u_target|^2, S_0 = feq(ρ0, u0), T steps of
+# collide! -> stream! on a periodic lattice (lbm_adjoint_*).  u0, u_target:
+# (nx, ny, 2) arrays.  τ: a number (SRT) or (τ_s, τ_a) (TRT).
+function with_adjoint(f, q, u0, τ, nsteps; storage = :f64, rho0 = 1.0, checkpoint_every = 0)
+    m = native_model(q)
+    cm = τ isa Number ? SRT(τ) : TRT(τ[1], τ[2], nothing)
+    set_collision!(m, cm)
+    set_force!(m, cm, size(u0, 1), size(u0, 2), 0.0)
+    set_bcs!(m, BoundaryCondition[])
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:lbm_adjoint_create, libpath), Cint,
+                (Ptr{Ptr{Cvoid}}, Ptr{Cvoid}, Cint, Cint, Cint, Cdouble, Int64, Cint),
+                h, m.handle, STORAGE[storage], size(u0, 1), size(u0, 2), rho0, nsteps,
+                checkpoint_every), "lbm_adjoint_create")
+    try
+        f(h[])
+    finally
+        ccall((:lbm_adjoint_destroy, libpath), Cint, (Ptr{Cvoid},), h[])
+    end
+end
+
+function simulate_loss(q::Quadrature, u0::Array{Float64, 3}, τ, u_target::Array{Float64, 3},
+                       nsteps::Integer; kw...)
+    with_adjoint(q, u0, τ, nsteps; kw...) do h
+        L = Ref(0.0)
+        GC.@preserve u0 u_target check(ccall((:lbm_adjoint_loss, libpath), Cint,
+                                             (Ptr{Cvoid}, Ptr{Cdouble}, Ptr{Cdouble},
+                                              Ptr{Cdouble}, Ptr{Cdouble}),
+                                             h, u0, u_target, L, C_NULL), "lbm_adjoint_loss")
+        L[]
+    end
+end
+
+function adjoint_gradient(q::Quadrature, u0::Array{Float64, 3}, τ, u_target::Array{Float64, 3},
+                          nsteps::Integer; kw...)
+    with_adjoint(q, u0, τ, nsteps; kw...) do h
+        L = Ref(0.0)
+        g = similar(u0)
+        gτ = zeros(2)
+        GC.@preserve u0 u_target g gτ check(ccall((:lbm_adjoint_gradient, libpath), Cint,
+                                                  (Ptr{Cvoid}, Ptr{Cdouble}, Ptr{Cdouble},
+                                                   Ptr{Cdouble}, Ptr{Cdouble}, Ptr{Cdouble},
+                                                   Ptr{Cdouble}),
+                                                  h, u0, u_target, L, g, gτ, C_NULL),
+                                            "lbm_adjoint_gradient")
+        L[], g, τ isa Number ? gτ[1] : (gτ[1], gτ[2])
+    end
+end
+
+function ChainRulesCore.rrule(::typeof(simulate_loss), q::Quadrature, u0, τ, u_target,
+                              nsteps::Integer; kw...)
+    L, g, gτ = adjoint_gradient(q, u0, τ, u_target, nsteps; kw...)
+    function simulate_loss_pullback(ΔL)
+        NoT = ChainRulesCore.NoTangent()
+        return (NoT, NoT, ΔL .* g, τ isa Number ? ΔL * gτ : (ΔL * gτ[1], ΔL * gτ[2]), NoT, NoT)
+    end
+    return L, simulate_loss_pullback
+end
+
+end # module

@@ -1,8 +1,8 @@
 """Boundary conditions (src/boundary_conditions.jl, boundary_conditions/*.jl).
 
-``BounceBack(direction, xs, ys)`` -- halfway bounce-back (bounce_back.jl:110-181);
+``BounceBack(direction, xs, ys)`` -- halfway bounce-back (bounce_back.jl:1-72);
 ``MovingWall(direction, xs, ys, u, ρ=1, T=1)`` -- bounce-back plus the wall
-momentum term 2 w_i s ρ_w u_w·c_i (moving_wall.jl:182-219; North only, and,
+momentum term 2 w_i s ρ_w u_w·c_i (moving_wall.jl:1-38; North only, and,
 as in the reference, it acts on the whole lattice regardless of xs, ys).
 Ranges are Julia UnitRanges: pass ``range(1, NX + 1)`` or an inclusive
 ``(first, last)`` pair, 1-based.
@@ -85,7 +85,7 @@ class MovingWall(BoundaryCondition):
 
     def __init__(self, direction, xs, ys, u: Sequence[float], ρ: float = 1.0, T: float = 1.0):
         if not isinstance(direction, North):
-            raise NotImplementedError("MovingWall is defined for North only (moving_wall.jl:198)")
+            raise NotImplementedError("MovingWall is defined for North only (moving_wall.jl:17)")
         object.__setattr__(self, "direction", direction)
         object.__setattr__(self, "xs", _unit_range(xs))
         object.__setattr__(self, "ys", _unit_range(ys))

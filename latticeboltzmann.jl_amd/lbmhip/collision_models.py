@@ -1,12 +1,12 @@
 """Collision models SRT, TRT, MRT (src/collision_models.jl, collision_models/*.jl).
 
 Constructors mirror the reference, including its argument-order quirk:
-``TRT(τ_a, τ_s)`` with two arguments swaps them (trt.jl:87), while the
+``TRT(τ_a, τ_s)`` with two arguments swaps them (trt.jl:6), while the
 three-field form ``TRT.fields(τ_s, τ_a, force)`` is the struct order
-(trt.jl:82-86).
+(trt.jl:1-5).
 
 Forces.  The reference stores a Julia closure ``(x_idx, y_idx, t) -> F`` and
-calls it per node (srt.jl:71).  A closure cannot cross the C ABI, so the host
+calls it per node (srt.jl:52-54).  A closure cannot cross the C ABI, so the host
 evaluates it into a lattice force field; a constant 2-vector or a prebuilt
 ``(nx, ny, 2)`` array can be given directly instead.
 """
@@ -26,7 +26,7 @@ class CollisionModel:
 
 @dataclass
 class SRT(CollisionModel):
-    """SRT{Force}(τ, force) -- srt.jl:20-24"""
+    """SRT{Force}(τ, force) -- srt.jl:1-5"""
     τ: float
     force: Any = None
 
@@ -37,9 +37,9 @@ class SRT(CollisionModel):
 
 @dataclass
 class TRT(CollisionModel):
-    """TRT{Force,T}(τ_symmetric, τ_asymmetric, force) -- trt.jl:82-86.
+    """TRT{Force,T}(τ_symmetric, τ_asymmetric, force) -- trt.jl:1-5.
 
-    ``TRT(a, b)`` follows trt.jl:87: ``TRT(τ_a, τ_s) = TRT(τ_s, τ_a, nothing)``.
+    ``TRT(a, b)`` follows trt.jl:6: ``TRT(τ_a, τ_s) = TRT(τ_s, τ_a, nothing)``.
     """
     τ_symmetric: float
     τ_asymmetric: float
@@ -60,14 +60,14 @@ class TRT(CollisionModel):
 
 
 def trt_magic(τ: float, Λ: float = 0.25, force=None) -> TRT:
-    """Magic-parameter factory: τ_s = τ, τ_a = 1/2 + Λ/(τ - 1/2) (trt.jl:89-102)."""
+    """Magic-parameter factory: τ_s = τ, τ_a = 1/2 + Λ/(τ - 1/2) (trt.jl:8-21)."""
     τ_ = 0.5 + Λ / (τ - 0.5)
     return TRT.fields(τ, τ_, force)
 
 
 @dataclass
 class TRT_Λ:
-    """TRT_Λ(Λ) -- trt.jl:116-121"""
+    """TRT_Λ(Λ) -- trt.jl:35-40"""
     Λ: float
 
 
@@ -95,7 +95,7 @@ class MRT(CollisionModel):
 
 def collision_model_from_viscosity(kind, q: Quadrature, ν: float, force=None, Λ: float = 0.25):
     """CollisionModel(SRT|TRT|MRT, q, problem) with τ = s ν + 1/2
-    (collision_models.jl:11-18, srt.jl:26-35, trt.jl:89-102, mrt.jl:36-47)."""
+    (collision_models.jl:11-18, srt.jl:7-16, trt.jl:8-21, mrt.jl:36-47)."""
     τ = q.speed_of_sound_squared * ν + 0.5
     if kind is SRT:
         return SRT(τ, force)

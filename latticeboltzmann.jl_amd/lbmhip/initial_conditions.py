@@ -38,6 +38,6 @@ def equilibrium_populations(q: Quadrature, rho, ux, uy) -> np.ndarray:
 
 
 def cell_centres(n: int, length: float = 1.0) -> np.ndarray:
-    """range(Δx/2, stop = L - Δx/2, length = n) -- problems.jl:128-136"""
+    """range(Δx/2, stop = L - Δx/2, length = n) -- problems.jl:18-26"""
     dx = length / n
     return np.linspace(dx / 2, length - dx / 2, n)

@@ -3,7 +3,7 @@
 ``moments(f, τ)`` returns per-node ρ, u and the deviatoric stress σ as the
 reference computes them per node in TrackHydrodynamicErrors
 (track_hydrodynamic_errors.jl:139-177): ``density`` / ``velocity!``
-(moments.jl:1-19) and ``deviatoric_tensor(q, τ, f, ρ, u)`` (moments.jl:199-214).
+(moments.jl:1-19) and ``deviatoric_tensor(q, τ, f, ρ, u)`` (moments.jl:81-96).
 ``conserved_sums(f)`` gives the FP64 mass / momentum / energy sums and the
 number of nodes holding a non-finite population (the NaN check of
 stopping_criteria.jl:47-50).

@@ -5,10 +5,10 @@ Julia's mutating ``collide!`` / ``stream!`` / ``apply!`` become ``collide_`` /
 Both the positional and the keyword forms of the reference are accepted:
 
 * ``collide_(cm, q, f_in, f_out, time=0.0)`` and
-  ``collide_(cm, q, f_old=..., f_new=..., time=...)`` (srt.jl:37-43,
-  trt.jl:123-129, mrt.jl:49, collision_models.jl:19);
+  ``collide_(cm, q, f_old=..., f_new=..., time=...)`` (srt.jl:18-24,
+  trt.jl:42-48, mrt.jl:49, collision_models.jl:19);
 * ``stream_(q, f, f_new)`` and ``stream_(q, f_new=..., f_old=...)`` (stream.jl:18-19);
-* ``apply_(bcs, q, f_new, f_old, time=0.0)`` (boundary_conditions.jl:80-94).
+* ``apply_(bcs, q, f_new, f_old, time=0.0)`` (boundary_conditions.jl:6-20).
 
 Each call launches the matching gfx950 kernel through liblbm_hip.so.
 """

@@ -104,7 +104,7 @@ def by_id(qid: int) -> Quadrature:
 
 
 # ---------------------------------------------------------------------------
-# Hermite tensors (src/hermite_polynomials.jl:124-159), built from the
+# Hermite tensors (src/hermite_polynomials.jl:47-82), built from the
 # Rodrigues-style recurrence H_{n+1} = xi (x) H_n - cs * sym(delta (x) H_{n-1})
 # rather than the reference's closed forms, as an independent formulation.
 # ---------------------------------------------------------------------------
@@ -260,7 +260,7 @@ def stream(q, f):
 
 
 def apply_bcs(q, bcs, fnew, fold):
-    """Ordered BC list with boolean masks (boundary_conditions.jl:80-90,
+    """Ordered BC list with boolean masks (boundary_conditions.jl:6-20,
     bounce_back.jl, moving_wall.jl).  bcs: dicts as in oracle.clib.make_bcs."""
     nx, ny, Q = fnew.shape
     out = fnew.copy()
@@ -303,7 +303,7 @@ def simulate(q, f_stream, collision, taus, nsteps, force=None, bcs=()):
 
 
 def deviatoric_field(q, f, tau):
-    """sigma = dev((sum_i f_i H2_i - rho u u) / (1 + 1/(2 tau))) (moments.jl:199-214)."""
+    """sigma = dev((sum_i f_i H2_i - rho u u) / (1 + 1/(2 tau))) (moments.jl:81-96)."""
     cs = 1.0 / q.s
     rho, ux, uy = moments_field(q, f)
     cc = q.c.astype(float)

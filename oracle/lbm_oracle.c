@@ -252,7 +252,7 @@ void lbo_equilibrium(const lbo_quad *q, double rho, const double *u, double *feq
 }
 
 /* ------------------------------------------------------------------------ */
-/* Tensor Hermite polynomials scaled by 1/s: src/hermite_polynomials.jl:124-159 */
+/* Tensor Hermite polynomials scaled by 1/s: src/hermite_polynomials.jl:47-82 */  
 /* A rank-n tensor over D=2 is flattened column-major (first index fastest). */
 /* ------------------------------------------------------------------------ */
 static int delta(int a, int b) { return a == b ? 1 : 0; }
@@ -363,7 +363,7 @@ static void node_force(int kind, const double *force, int nx, int ny, int x, int
     }
 }
 
-/* collide!(::SRT) -- src/collision_models/srt.jl:37-81 */
+/* collide!(::SRT) -- src/collision_models/srt.jl:18-62 */
 void lbo_collide_srt(const lbo_quad *q, int nx, int ny, const double *fin, double *fout,
                      double tau, int force_kind, const double *force) {
     const int Q = q->Q;
@@ -377,7 +377,7 @@ void lbo_collide_srt(const lbo_quad *q, int nx, int ny, const double *fin, doubl
             if (force_kind != LBO_FORCE_NONE) {
                 double F[2];
                 node_force(force_kind, force, nx, ny, x, y, F);
-                double v[2] = {u[0] + tau * F[0], u[1] + tau * F[1]}; /* srt.jl:73 */
+                double v[2] = {u[0] + tau * F[0], u[1] + tau * F[1]}; /* srt.jl:54 */
                 lbo_equilibrium(q, rho, v, feq);
             } else {
                 lbo_equilibrium(q, rho, u, feq);
@@ -387,7 +387,7 @@ void lbo_collide_srt(const lbo_quad *q, int nx, int ny, const double *fin, doubl
         }
 }
 
-/* collide!(::TRT) -- src/collision_models/trt.jl:123-178 */
+/* collide!(::TRT) -- src/collision_models/trt.jl:42-97 */
 void lbo_collide_trt(const lbo_quad *q, int nx, int ny, const double *fin, double *fout,
                      double tau_s, double tau_a, int force_kind, const double *force) {
     const int Q = q->Q;
@@ -401,7 +401,7 @@ void lbo_collide_trt(const lbo_quad *q, int nx, int ny, const double *fin, doubl
             if (force_kind != LBO_FORCE_NONE) {
                 double F[2];
                 node_force(force_kind, force, nx, ny, x, y, F);
-                double v[2] = {u[0] + tau_a * F[0], u[1] + tau_a * F[1]}; /* trt.jl:160 */
+                double v[2] = {u[0] + tau_a * F[0], u[1] + tau_a * F[1]}; /* trt.jl:79 */
                 lbo_equilibrium(q, rho, v, feq);
             } else {
                 lbo_equilibrium(q, rho, u, feq);
@@ -490,9 +490,9 @@ void lbo_stream(const lbo_quad *q, int nx, int ny, const double *f, double *fnew
 }
 
 /* ------------------------------------------------------------------------ */
-/* apply!(bcs, ...) -- src/boundary_conditions.jl:80-90; BounceBack          */
-/* boundary_conditions/bounce_back.jl:117-181; MovingWall{North}             */
-/* boundary_conditions/moving_wall.jl:198-219                                */
+/* apply!(bcs, ...) -- src/boundary_conditions.jl:6-20; BounceBack           */
+/* boundary_conditions/bounce_back.jl:8-72; MovingWall{North}                */
+/* boundary_conditions/moving_wall.jl:17-38                                  */
 /* ------------------------------------------------------------------------ */
 int lbo_apply_bcs(const lbo_quad *q, int nx, int ny, const lbo_bc *bcs, int nbc,
                   double *fnew, const double *fold) {
@@ -570,7 +570,7 @@ int lbo_simulate(const lbo_quad *q, int nx, int ny, double *f_stream, double *f_
 
 /* ------------------------------------------------------------------------ */
 /* Moment fields: density/velocity! (moments.jl:1-19) and deviatoric_tensor  */
-/* (moments.jl:199-214) as used by TrackHydrodynamicErrors                   */
+/* (moments.jl:81-96) as used   by TrackHydrodynamicErrors                   */
 /* (processing_methods/track_hydrodynamic_errors.jl:139-177)                 */
 /* ------------------------------------------------------------------------ */
 void lbo_moments(const lbo_quad *q, int nx, int ny, const double *fin, double tau,

@@ -28,10 +28,10 @@ def bc_dicts(name: str, nx: int, ny: int, u_wall=(0.05, 0.0)):
         return []
     if name == "poiseuille":   # poiseuille.jl:93-96
         return [dict(kind=0, direction=0, **full), dict(kind=0, direction=2, **full)]
-    if name == "couette":      # couette_flow.jl:178-181
+    if name == "couette":      # couette_flow.jl:82-85
         return [dict(kind=0, direction=2, **full),
                 dict(kind=1, direction=0, u=u_wall, rho=1.0, **full)]
-    if name == "cavity":       # lid_driven_cavity.jl:242-247
+    if name == "cavity":       # lid_driven_cavity.jl:61-66
         return [dict(kind=0, direction=1, **full), dict(kind=0, direction=2, **full),
                 dict(kind=0, direction=3, **full),
                 dict(kind=1, direction=0, u=u_wall, rho=1.0, **full)]

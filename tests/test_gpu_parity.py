@@ -127,7 +127,7 @@ def test_simulate_bit_exact(oracle_lib, qid, coll, bcname):
 @pytest.mark.parametrize("shape", [(1, 5), (3, 5), (2, 2), (1, 1), (5, 3)])
 def test_tiny_lattices(oracle_lib, qid, shape):
     """|c| >= n wraps (stream.jl:69-74) and BCs hitting most rows (nx = 1 as
-    CouetteFlow, couette_flow.jl:115; NX = 3 as PoiseuilleFlow, poiseuille.jl:22)."""
+    CouetteFlow, couette_flow.jl:19; NX = 3 as PoiseuilleFlow, poiseuille.jl:22)."""
     L = _L()
     nx, ny = shape
     f0 = random_populations(qid, nx, ny, seed=11)

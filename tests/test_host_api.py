@@ -4,7 +4,7 @@ import pytest
 
 
 def test_trt_two_argument_swap():
-    """TRT(a, b) = TRT(tau_s = b, tau_a = a) (trt.jl:87)."""
+    """TRT(a, b) = TRT(tau_s = b, tau_a = a) (trt.jl:6)."""
     import lbmhip as L
     t = L.TRT(0.9, 1.1)
     assert t.τ_symmetric == 1.1 and t.τ_asymmetric == 0.9
@@ -13,7 +13,7 @@ def test_trt_two_argument_swap():
 
 
 def test_trt_magic_parameter():
-    """tau_a = 1/2 + Lambda / (tau - 1/2) (trt.jl:95-96)."""
+    """tau_a = 1/2 + Lambda / (tau - 1/2) (trt.jl:14-15)."""
     import lbmhip as L
     t = L.trt_magic(0.6, 0.25)
     assert t.τ_symmetric == 0.6 and t.τ_asymmetric == pytest.approx(3.0)
@@ -22,7 +22,7 @@ def test_trt_magic_parameter():
 
 
 def test_collision_model_from_viscosity():
-    """tau = s nu + 1/2 (srt.jl:26-35, trt.jl:89-102, mrt.jl:36-47)."""
+    """tau = s nu + 1/2 (srt.jl:7-16, trt.jl:8-21, mrt.jl:36-47)."""
     import lbmhip as L
     q = L.D2Q9()
     assert L.collision_model_from_viscosity(L.SRT, q, 0.1).τ == pytest.approx(0.8)

@@ -1,10 +1,10 @@
 """Pins the oracle's collision operators.
 
-* SRT at tau = 1 returns the equilibrium (test/quadrature.jl:183-215);
+* SRT at tau = 1 returns the equilibrium (test/quadrature.jl:64-96);
 * TRT(tau, tau) == SRT(tau), on the uniform field and over 11 Poiseuille
-  steps for tau in 0.51..1.1 (test/collision_models.jl:269-321);
+  steps for tau in 0.51..1.1 (test/collision_models.jl:3-55);
 * MRT(q, tau) == SRT(tau) on the uniform field, and over the Poiseuille steps
-  at tau = 1 (test/collision_models.jl:323-380);
+  at tau = 1 (test/collision_models.jl:57-114);
 * conservation: mass always, momentum without force, momentum gain rho F
   per collision with the velocity-shift force of SRT and TRT;
 * the C restatement against the independent NumPy formulation.
@@ -48,7 +48,7 @@ def test_trt_equal_times_is_srt(oracle_lib, qid):
 
 def _poiseuille_init(qid, NX=3, NY=5, scale=1):
     """AnalyticalEquilibrium of PoiseuilleFlow(nu, scale) (poiseuille.jl:12-26,
-    48-58; problems.jl:128-136): rho = 1, u_x = u_max y (1 - y) G / 2."""
+    48-58; problems.jl:18-26): rho = 1, u_x = u_max y (1 - y) G / 2."""
     q = ln.by_id(qid)
     u_max = 0.1 / scale
     y = (np.arange(NY) + 0.5) / NY

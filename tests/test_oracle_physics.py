@@ -3,7 +3,7 @@
 * C1 shear wave: D2Q9 SRT, 64 x 64, tau = 0.8, 1000 steps; the mode decays as
   exp(-nu k^2 t) (decaying_shear_flow.jl:76-93), mass is conserved;
 * Couette flow is exact on D2Q9 (README.md: "Couette flow is exact"; BCs of
-  couette_flow.jl:174-181): linear profile to rounding;
+  couette_flow.jl:82-85): linear profile to rounding;
 * Poiseuille flow (poiseuille.jl:48-58, 72-96): parabolic steady profile;
 * Taylor-Green vortex decay exp(-2 nu k^2 t) with TRT (second_order_convergence.jl:68-84);
 * second-order convergence of the shear wave under diffusive scaling (the

@@ -1,7 +1,7 @@
 """Pins the oracle's velocity sets (quadrature constants and opposite maps).
 
-Restates test/quadrature.jl:120-138 (sum w = 1, sum_i w_i H_n(c_i) = 0 for
-n <= order/2), test/quadrature.jl:243-249 (c_i + c_opp(i) = 0), and checks
+Restates test/quadrature.jl:1-19 (sum w = 1, sum_i w_i H_n(c_i) = 0 for
+n <= order/2), test/quadrature.jl:124-130 (c_i + c_opp(i) = 0), and checks
 the C restatement against the independently transcribed NumPy tables.
 """
 import numpy as np
@@ -47,7 +47,7 @@ def test_opposite(oracle_lib, qid):
 
 
 def test_d2q9_known_values(oracle_lib):
-    """D2Q9.jl:58-76 spelled out."""
+    """D2Q9.jl:20-38 spelled out."""
     c, w, s, order, opp, _ = oracle_lib.quad_arrays(9)
     assert c.tolist() == [[0, -1, -1, -1, 0, 1, 1, 1, 0], [0, 1, 0, -1, -1, -1, 0, 1, 1]]
     assert w.tolist() == [4 / 9, 1 / 36, 1 / 9, 1 / 36, 1 / 9, 1 / 36, 1 / 9, 1 / 36, 1 / 9]
@@ -64,7 +64,7 @@ def test_d2q21_zero_weight_planes(oracle_lib):
 
 def test_multispeed_isotropy_pins_s(oracle_lib):
     """A plausible transcription mistake in s (e.g. D2Q21's overridden
-    cs = sqrt(2/3) of D2Q21.jl:106) fails the second-order isotropy check."""
+    cs = sqrt(2/3) of D2Q21.jl:16) fails the second-order isotropy check."""
     c, w, s, *_ = oracle_lib.quad_arrays(21)
     second = np.einsum("i,ai,bi->ab", w, c.astype(float), c.astype(float))
     np.testing.assert_allclose(second, np.eye(2) / s, atol=1e-15)

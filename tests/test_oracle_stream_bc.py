@@ -1,12 +1,12 @@
 """Pins the oracle's streaming and boundary conditions.
 
 * stream! is a per-plane periodic permutation: equal to numpy.roll by c_i,
-  for every lattice size including |c| >= n (test/quadrature.jl:251-266);
+  for every lattice size including |c| >= n (test/quadrature.jl:135-147);
 * collide(tau = 1) + stream on a 1x1 equilibrium is the identity
-  (test/quadrature.jl:217-240);
+  (test/quadrature.jl:98-121);
 * halfway bounce-back on a one-row lattice maps i -> opp(i);
 * ordered BC lists (last writer wins) against the NumPy mask formulation;
-* the moving-wall term 2 w_i s rho_w u_w . c_i (moving_wall.jl:198-219).
+* the moving-wall term 2 w_i s rho_w u_w . c_i (moving_wall.jl:17-38).
 """
 import numpy as np
 import pytest
