@@ -183,6 +183,77 @@ def adjoint_leg(L, n=512, T=2000, tau=0.7):
     return out
 
 
+def startup_profile(flow, ny, nu, t, F=1e-6, u_wall=0.05, nterms=4000):
+    """Analytic start-up of plane Poiseuille (body force F, walls at rest) and
+    Couette (top wall u_wall) flow from rest between halfway walls at y = 0 and
+    y = ny (node centres y_j = j + 1/2), after t steps of viscosity nu:
+    Fourier sine series of the diffusion equation."""
+    h = float(ny)
+    y = np.arange(ny) + 0.5
+    n = np.arange(1, nterms + 1, dtype=np.float64)
+    decay = np.exp(-nu * (n * np.pi / h) ** 2 * t)
+    modes = np.sin(np.outer(n, np.pi * y / h))
+    if flow == "poiseuille":
+        coef = np.where(n % 2 == 1, 4.0 * F * h * h / (nu * (n * np.pi) ** 3), 0.0)
+        return (coef * (1.0 - decay)) @ modes
+    coef = 2.0 * u_wall * (-1.0) ** n / (n * np.pi)
+    return u_wall * y / h + (coef * decay) @ modes
+
+
+def precision_leg(L, steps, nx=8192, ny=2048, tau=0.8, qids=(9, 17, 37),
+                  flows=("poiseuille", "couette"),
+                  storages=("f64", "f32", "f16dev", "bf16dev", "f16neq", "bf16neq")):
+    """D2Q9 / D2Q17 / D2Q37 SRT(tau) on an nx x ny channel (periodic in x,
+    walls at the bottom and top rows), from rest: Poiseuille with the constant
+    body force (1e-6, 0) and bounce-back walls (poiseuille.jl:93-96), Couette
+    with a bounce-back bottom and a top wall moving at (0.05, 0)
+    (couette_flow.jl:82-85).  Each storage runs the same steps; the velocity
+    field is compared with the FP64 run (relative L2 over the lattice) and with
+    the analytic start-up profile."""
+    rows = []
+    for qid in qids:
+        q = L.Quadratures[f"D2Q{qid}"]()
+        nu = (tau - 0.5) / q.speed_of_sound_squared
+        f0 = L.equilibrium_populations(q, np.ones((nx, ny)), np.zeros((nx, ny)),
+                                       np.zeros((nx, ny)))
+        for flow in flows:
+            full = ((1, nx), (1, ny))
+            if flow == "poiseuille":
+                cm = L.SRT(tau, np.array([1e-6, 0.0]))
+                bcs = [L.BounceBack(L.North(), *full), L.BounceBack(L.South(), *full)]
+            else:
+                cm = L.SRT(tau)
+                bcs = [L.BounceBack(L.South(), *full),
+                       L.MovingWall(L.North(), *full, (0.05, 0.0))]
+            ua = startup_profile(flow, ny, nu, steps)
+            ref = None
+            for st in storages:
+                m = L.LatticeBoltzmannModel.from_populations(q, f0, cm, bcs, storage=st)
+                L.synchronize()
+                t0 = time.perf_counter()
+                kms, nl = L.simulate_timed(m, steps)
+                L.synchronize()
+                wall = time.perf_counter() - t0
+                mom = L.moments(m.f_stream, stress=False)
+                ux, uy = mom.ux, mom.uy
+                if st == "f64":
+                    ref = (ux, uy)
+                den = np.sqrt(np.sum(ref[0] ** 2 + ref[1] ** 2))
+                e64 = float(np.sqrt(np.sum((ux - ref[0]) ** 2 + (uy - ref[1]) ** 2)) / den)
+                ea = float(np.sqrt(np.sum((ux - ua[None, :]) ** 2 + uy ** 2))
+                           / np.sqrt(nx * np.sum(ua ** 2)))
+                row = {"quadrature": f"D2Q{qid}", "flow": flow, "storage": st, "steps": steps,
+                       "rel_l2_vs_f64": e64, "rel_l2_vs_analytic": ea,
+                       "mlups": round(nx * ny * steps / wall / 1e6, 1),
+                       "fused_kernel_ms": round(kms / max(nl, 1), 5),
+                       "u_max": float(np.abs(ux).max())}
+                rows.append(row)
+                print(json.dumps(row), flush=True)
+                del m
+        del f0
+    return rows
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -195,6 +266,10 @@ def main():
     ap.add_argument("--no-fp16", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--precision", action="store_true",
+                    help="run only the 8192x2048 channel precision table (minutes)")
+    ap.add_argument("--precision-steps", type=int, default=20000)
+    ap.add_argument("--precision-quadratures", default="9,17,37")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
@@ -205,6 +280,11 @@ def main():
     if L.device_count() < 1:
         print("bench.py: no GPU visible", file=sys.stderr)
         sys.exit(1)
+    if a.precision:
+        qids = tuple(int(v) for v in a.precision_quadratures.split(","))
+        rows = precision_leg(L, a.precision_steps, qids=qids)
+        print(json.dumps({"precision": rows}), flush=True)
+        return
     n, K, W = a.n, a.steps, a.warmup
     q = L.D2Q9()
     cm = L.trt_magic(a.tau, 0.25)
@@ -259,8 +339,8 @@ def main():
 
     if not a.no_fp16:
         # 16-bit storages: rest-state deviations (f16dev) and non-equilibrium
-        # parts about FP32 moments (f16neq: 2 Q + 16 bytes per node)
-        for storage, key, per_node in (("f16neq", "fp16neq", 2 * q.Q + 16),
+        # parts about a binary16 equilibrium record (f16neq: 2 Q + 8 bytes per node)
+        for storage, key, per_node in (("f16neq", "fp16neq", 2 * q.Q + 8),
                                        ("f16dev", "fp16dev", 2 * q.Q)):
             print(f"[bench] {storage}: {W} warmup + {K} timed steps", file=sys.stderr,
                   flush=True)

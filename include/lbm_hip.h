@@ -37,10 +37,10 @@ enum { LBM_D2Q4 = 4, LBM_D2Q5 = 5, LBM_D2Q9 = 9, LBM_D2Q13 = 13,
  *   F32       populations in binary32;
  *   F16DEV    deviations f_i - w_i rho0 in binary16;
  *   BF16DEV   the same in bfloat16;
- *   F16NEQ    per node a 16-byte record of FP32 moments (rho - rho0, ux, uy)
- *             and, per population, the non-equilibrium part f_i - feq_i(rho, u)
- *             in binary16 (scaled by 2^16);
- *   BF16NEQ   the same in bfloat16 (unscaled).
+ *   F16NEQ    per node an 8-byte record of four binary16 equilibrium
+ *             coefficients (from rho - rho0 and u) and, per population, the
+ *             non-equilibrium part f_i - feq_i(record) in binary16 (scaled by 2^16);
+ *   BF16NEQ   the same with bfloat16 non-equilibrium parts (unscaled).
  * F32 / *DEV / *NEQ arithmetic runs on g_i = f_i - w_i rho0 (rho0 of the field). */
 enum { LBM_STORE_F64 = 0, LBM_STORE_F32 = 1, LBM_STORE_F16DEV = 2, LBM_STORE_BF16DEV = 3,
        LBM_STORE_F16NEQ = 4, LBM_STORE_BF16NEQ = 5 };

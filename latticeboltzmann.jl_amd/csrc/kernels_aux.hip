@@ -273,8 +273,8 @@ __device__ __forceinline__ void geq_rec_all(const Mom4 &m, const StepParams<floa
     geq_rec_seq<QID>(m, p, geq, std::make_integer_sequence<int, QT<QID>::Q>{});
 }
 
-// Float64 (nx, ny, Q) <-> non-equilibrium storage (moments in FP32, feq in FP32
-// with the kernels' formula, 16-bit non-equilibrium parts)
+// Float64 (nx, ny, Q) <-> non-equilibrium storage (binary16 equilibrium record,
+// feq in FP32 with the kernels' formula, 16-bit non-equilibrium parts)
 template <int QID, class ST>
 __global__ void __launch_bounds__(256)
 k_neq_codec(double *__restrict__ hostlike, typename ST::store_t *dev, int to_device,
@@ -282,7 +282,7 @@ k_neq_codec(double *__restrict__ hostlike, typename ST::store_t *dev, int to_dev
     constexpr int Q = QT<QID>::Q;
     const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= pf.plane) return;
-    Mom4 *m = ST::mom(dev);
+    Rec16 *m = ST::recs(dev);
     auto *P = ST::pops(dev, pf.plane);
     if (to_device) {
         double f[Q];
@@ -298,16 +298,16 @@ k_neq_codec(double *__restrict__ hostlike, typename ST::store_t *dev, int to_dev
             my += g[i] * QT<QID>::cy(i);
         }
         const double rho = (double)pf.rho0 + drho;
-        const Mom4 rec = make_rec<QID>((float)drho, (float)(mx / rho), (float)(my / rho), pf);
+        const Rec16 rec = ST::pack(make_rec<QID>((float)drho, (float)(mx / rho), (float)(my / rho), pf));
         m[n] = rec;
         float geq[Q];
-        geq_rec_all<QID>(rec, pf, geq);
+        geq_rec_all<QID>(ST::unpack(rec), pf, geq);
 #pragma unroll
         for (int i = 0; i < Q; ++i)
             P[pf.plane * i + n] = ST::from_f((float)(g[i] - (double)geq[i]), pf);
     } else {
         float geq[Q];
-        geq_rec_all<QID>(m[n], pf, geq);
+        geq_rec_all<QID>(ST::unpack(m[n]), pf, geq);
 #pragma unroll
         for (int i = 0; i < Q; ++i)
             hostlike[pf.plane * i + n] = (double)pf.w[i] * (double)pf.rho0 + (double)geq[i] +

@@ -430,7 +430,7 @@ int lbm_field_create(lbm_field *out, int quadrature, int nx, int ny, int storage
     f->dscale = storage == LBM_STORE_F16DEV ? 256.0 : (storage == LBM_STORE_F16NEQ ? 65536.0 : 1.0);
     f->bytes = (size_t)nx * (size_t)ny * (size_t)q.Q * (size_t)eb;
     if (storage == LBM_STORE_F16NEQ || storage == LBM_STORE_BF16NEQ)
-        f->bytes += (size_t)nx * (size_t)ny * 4 * sizeof(float);  // 16-byte moment records
+        f->bytes += (size_t)nx * (size_t)ny * 8;  // 8-byte equilibrium records
     hipError_t e = hipMalloc(&f->d, f->bytes);
     if (e != hipSuccess) {
         delete f;

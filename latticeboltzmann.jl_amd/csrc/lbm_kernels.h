@@ -82,20 +82,28 @@ struct StBF16Dev : StBase<StBF16Dev, __hip_bfloat16, float> {
     }
 };
 
-// Non-equilibrium storage: per node one 16-byte equilibrium record (below)
+// Non-equilibrium storage: per node one 8-byte equilibrium record (below)
 // and, per population, the 16-bit non-equilibrium part
 // (f_i - feq_i) * dscale; the arithmetic value is
 // g_i = f_i - w_i rho0 = (feq_i - w_i rho0) + (f_i - feq_i).
-// The field holds the node records first (one dwordx4 per neighbour read),
+// The field holds the node records first (one dwordx2 per neighbour read),
 // then Q 16-bit planes.
 //
 // Equilibrium record.  For first- and second-order equilibria it holds
 //   a = (drho - s rho |u|^2 / 2,  s rho ux,  s rho uy,  1 / (2 rho))
 // so that feq_i - w_i rho0 = w_i (a0 + c_i.A + a3 (c_i.A)^2), A = (a1, a2):
 // six operations per population instead of the full polynomial.  For the
-// third- and fourth-order sets it holds (drho, ux, uy, 0).
+// third- and fourth-order sets it holds (drho, ux, uy, 0).  In memory the
+// record is four binary16 values (a0 scaled by 2^10).  The stored
+// non-equilibrium parts are taken against the rounded record, so the record's
+// rounding is carried exactly by them and costs no accuracy beyond the
+// rounding of slightly larger non-equilibrium parts.
 struct alignas(16) Mom4 {
     float a0, a1, a2, a3;
+};
+
+struct alignas(8) Rec16 {
+    __half a0, a1, a2, a3;
 };
 
 template <class S, int ID>
@@ -105,6 +113,7 @@ struct StNeq {
     static constexpr int id = ID;
     static constexpr bool kNeq = true;
     static constexpr bool kShift = true;
+    static constexpr float kA0 = 1024.0f;
     __device__ static float to_f(S v, const StepParams<float> &q) {
         if constexpr (ID == 4) return __half2float(v) * q.dscale_inv;
         else return __bfloat162float(v) * q.dscale_inv;
@@ -113,11 +122,24 @@ struct StNeq {
         if constexpr (ID == 4) return __float2half_rn(v * q.dscale);
         else return __float2bfloat16(v * q.dscale);
     }
-    __device__ static const Mom4 *mom(const S *base) { return reinterpret_cast<const Mom4 *>(base); }
-    __device__ static Mom4 *mom(S *base) { return reinterpret_cast<Mom4 *>(base); }
-    // population plane 0 starts after the 16-byte records = 8 * plane 16-bit words
-    __device__ static const S *pops(const S *base, long long plane) { return base + 8 * plane; }
-    __device__ static S *pops(S *base, long long plane) { return base + 8 * plane; }
+    // an already scaled value to storage
+    __device__ static S raw(float v) {
+        if constexpr (ID == 4) return __float2half_rn(v);
+        else return __float2bfloat16(v);
+    }
+    __device__ static Rec16 pack(const Mom4 &m) {
+        return Rec16{__float2half_rn(m.a0 * kA0), __float2half_rn(m.a1), __float2half_rn(m.a2),
+                     __float2half_rn(m.a3)};
+    }
+    __device__ static Mom4 unpack(const Rec16 &r) {
+        return Mom4{__half2float(r.a0) * (1.0f / kA0), __half2float(r.a1), __half2float(r.a2),
+                    __half2float(r.a3)};
+    }
+    __device__ static const Rec16 *recs(const S *base) { return reinterpret_cast<const Rec16 *>(base); }
+    __device__ static Rec16 *recs(S *base) { return reinterpret_cast<Rec16 *>(base); }
+    // population plane 0 starts after the 8-byte records = 4 * plane 16-bit words
+    __device__ static const S *pops(const S *base, long long plane) { return base + 4 * plane; }
+    __device__ static S *pops(S *base, long long plane) { return base + 4 * plane; }
 };
 using StF16Neq = StNeq<__half, 4>;
 using StBF16Neq = StNeq<__hip_bfloat16, 5>;
@@ -228,7 +250,7 @@ __device__ __forceinline__ typename ST::T ld_pop(const typename ST::store_t *__r
                                                  const StepParams<typename ST::T> &p,
                                                  long long n) {
     if constexpr (ST::kNeq) {
-        return geq_rec<QID, I>(ST::mom(F)[n], p) +
+        return geq_rec<QID, I>(ST::unpack(ST::recs(F)[n]), p) +
                ST::to_f(ST::pops(F, p.plane)[p.plane * I + n], p);
     } else {
         return ST::load(F + p.plane * I, n, I, p);
@@ -269,6 +291,11 @@ __device__ __forceinline__ void feq_dev_all(T drho, T ux, T uy, const StepParams
     feq_dev_all_seq<QID, T>(drho, ux, uy, p, geq, std::make_integer_sequence<int, QT<QID>::Q>{});
 }
 
+// 1/x: the hardware reciprocal (1 ulp) for the FP32-arithmetic storages, the
+// correctly rounded quotient in FP64
+__device__ __forceinline__ float rcp_fast(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double rcp_fast(double x) { return 1.0 / x; }
+
 // equilibrium record of (drho, u) for quadrature QID (see Mom4)
 template <int QID>
 __device__ __forceinline__ Mom4 make_rec(float drho, float ux, float uy, const StepParams<float> &p) {
@@ -277,7 +304,7 @@ __device__ __forceinline__ Mom4 make_rec(float drho, float ux, float uy, const S
         const float rho = p.rho0 + drho;
         const float sr = p.s * rho;
         const float a0 = FEQ == 2 ? drho - 0.5f * sr * (ux * ux + uy * uy) : drho;
-        return Mom4{a0, sr * ux, sr * uy, FEQ == 2 ? 0.5f / rho : 0.0f};
+        return Mom4{a0, sr * ux, sr * uy, FEQ == 2 ? 0.5f * rcp_fast(rho) : 0.0f};
     } else {
         return Mom4{drho, ux, uy, 0.0f};
     }
@@ -308,8 +335,9 @@ __device__ __forceinline__ void moments_dev_seq(const T (&g)[QT<QID>::Q], const 
     ((my = (QT<QID>::cy(Is) != 0 ? my + cmul<QT<QID>::cy(Is)>(g[Is]) : my)), ...);
     const T rho = p.rho0 + r;
     drho = r;
-    ux = mx / rho;
-    uy = my / rho;
+    const T ir = rcp_fast(rho);
+    ux = mx * ir;
+    uy = my * ir;
 }
 
 template <int QID, class T>
@@ -388,6 +416,13 @@ __device__ __forceinline__ void collide_seq(T (&f)[QT<QID>::Q], const StepParams
     if constexpr (COLL == 0) {
         // srt.jl:57-59
         ((f[Is] = p.omo * f[Is] + p.om * feq[Is]), ...);
+    } else if constexpr (SHIFT) {
+        // trt.jl:82-94 rewritten on n = f - feq (FP32 storages): with
+        // n+- = (n_i +- n_opp)/2, f_i - (n+/tau_s + n-/tau_a) = feq_i + ka n_i + kb n_opp
+        const T ka = T(1) - T(0.5) * (p.oms + p.oma), kb = T(0.5) * (p.oma - p.oms);
+        T n[Q];
+        ((n[Is] = f[Is] - feq[Is]), ...);
+        ((f[Is] = feq[Is] + (ka * n[Is] + kb * n[QT<QID>::opp(Is)])), ...);
     } else {
         // trt.jl:82-94 (each population reads the pre-collision pair)
         T g[Q];
@@ -402,6 +437,48 @@ __device__ __forceinline__ void collide_seq(T (&f)[QT<QID>::Q], const StepParams
             f[i] = g[i] + ((-p.oms) * (f_s - feq_s) - p.oma * (f_a - feq_a));
         };
         (trt(std::integral_constant<int, Is>{}), ...);
+    }
+}
+
+// Collision for the non-equilibrium storages, done on n = f - feq: the
+// relaxed part is (1 - 1/tau) n_i (SRT) or ka n_i + kb n_opp(i) (TRT, as
+// above); the stored part adds the difference between the exact equilibrium
+// and that of the rounded record.  Returned multiplied by the storage scale,
+// with the rounded record of the node.
+template <int QID, int COLL, class ST, int... Is>
+__device__ __forceinline__ void collide_neq_seq(const float (&f)[QT<QID>::Q],
+                                                const StepParams<float> &p, int x, int y,
+                                                Rec16 &rec, float (&nq)[QT<QID>::Q],
+                                                std::integer_sequence<int, Is...>) {
+    constexpr int Q = QT<QID>::Q;
+    float drho, ux, uy;
+    node_moments_dev<QID, float>(f, p, drho, ux, uy);
+    if (p.force_kind != 0) {
+        float Fx, Fy;
+        node_force(p, x, y, Fx, Fy);
+        ux = ux + p.tau_force * Fx;
+        uy = uy + p.tau_force * Fy;
+    }
+    // relax towards the exact equilibrium; store against the rounded record:
+    // C_i - feq_i(record) = relax(n)_i + (feq_i(exact) - feq_i(record))
+    const Mom4 mt = make_rec<QID>(drho, ux, uy, p);
+    rec = ST::pack(mt);
+    const Mom4 mq = ST::unpack(rec);
+    float n[Q], d[Q];
+    auto split = [&](auto Ic) {
+        constexpr int i = decltype(Ic)::value;
+        const float ge = geq_rec<QID, i>(mt, p);
+        n[i] = f[i] - ge;
+        d[i] = (ge - geq_rec<QID, i>(mq, p)) * p.dscale;
+    };
+    (split(std::integral_constant<int, Is>{}), ...);
+    if constexpr (COLL == 0) {
+        const float k = p.omo * p.dscale;
+        ((nq[Is] = k * n[Is] + d[Is]), ...);
+    } else {
+        const float ka = (1.0f - 0.5f * (p.oms + p.oma)) * p.dscale;
+        const float kb = (0.5f * (p.oma - p.oms)) * p.dscale;
+        ((nq[Is] = (ka * n[Is] + kb * n[QT<QID>::opp(Is)]) + d[Is]), ...);
     }
 }
 
@@ -542,24 +619,24 @@ __device__ __forceinline__ void load_local(const typename ST::store_t *__restric
     ((f[Is] = ld_pop<QID, Is, ST>(F, p, node)), ...);
 }
 
-// moments of a population vector and its non-equilibrium parts (FP32)
-// moments (rho - rho0, u) of a shifted population vector and its
-// non-equilibrium parts f_i - feq_i
-template <int QID, int... Is>
+// the rounded equilibrium record of a shifted population vector and its
+// non-equilibrium parts f_i - feq_i(record)
+template <int QID, class ST, int... Is>
 __device__ __forceinline__ void neq_split_seq(const float (&f)[QT<QID>::Q],
-                                              const StepParams<float> &p, Mom4 &rec,
+                                              const StepParams<float> &p, Rec16 &rec,
                                               float (&g)[QT<QID>::Q],
                                               std::integer_sequence<int, Is...>) {
     float drho, ux, uy;
     node_moments_dev<QID, float>(f, p, drho, ux, uy);
-    rec = make_rec<QID>(drho, ux, uy, p);
-    ((g[Is] = f[Is] - geq_rec<QID, Is>(rec, p)), ...);
+    rec = ST::pack(make_rec<QID>(drho, ux, uy, p));
+    const Mom4 m = ST::unpack(rec);
+    ((g[Is] = f[Is] - geq_rec<QID, Is>(m, p)), ...);
 }
 
-template <int QID>
+template <int QID, class ST>
 __device__ __forceinline__ void neq_split(const float (&f)[QT<QID>::Q], const StepParams<float> &p,
-                                          Mom4 &rec, float (&g)[QT<QID>::Q]) {
-    neq_split_seq<QID>(f, p, rec, g, std::make_integer_sequence<int, QT<QID>::Q>{});
+                                          Rec16 &rec, float (&g)[QT<QID>::Q]) {
+    neq_split_seq<QID, ST>(f, p, rec, g, std::make_integer_sequence<int, QT<QID>::Q>{});
 }
 
 template <int QID, class ST, int... Is>
@@ -568,10 +645,10 @@ __device__ __forceinline__ void store_local(typename ST::store_t *__restrict__ F
                                             const typename ST::T (&f)[QT<QID>::Q],
                                             std::integer_sequence<int, Is...>) {
     if constexpr (ST::kNeq) {
-        Mom4 m;
+        Rec16 m;
         float g[QT<QID>::Q];
-        neq_split<QID>(f, p, m, g);
-        ST::mom(F)[node] = m;
+        neq_split<QID, ST>(f, p, m, g);
+        ST::recs(F)[node] = m;
         auto *P = ST::pops(F, p.plane);
         ((P[p.plane * Is + node] = ST::from_f(g[Is], p)), ...);
     } else {
@@ -655,11 +732,54 @@ __device__ __forceinline__ void pull_interior_vec(const typename ST::store_t *__
         Pack<S, VEC> pk;
         if constexpr (ST::kNeq) {
             __builtin_memcpy(&pk, ST::pops(C, p.plane) + p.plane * i + src, sizeof(pk));
-            const Mom4 *m = ST::mom(C) + src;
+            const Rec16 *m = ST::recs(C) + src;
 #pragma unroll
-            for (int v = 0; v < VEC; ++v) f[v][i] = geq_rec<QID, i>(m[v], p) + ST::to_f(pk.v[v], p);
+            for (int v = 0; v < VEC; ++v)
+                f[v][i] = geq_rec<QID, i>(ST::unpack(m[v]), p) + ST::to_f(pk.v[v], p);
         } else {
             __builtin_memcpy(&pk, C + p.plane * i + src, sizeof(pk));
+#pragma unroll
+            for (int v = 0; v < VEC; ++v) f[v][i] = ST::decode(pk.v[v], i, p);
+        }
+    };
+    (one(std::integral_constant<int, Is>{}), ...);
+}
+
+// Interior pull with wave-uniform row bases.  Every wavefront lies in one
+// lattice row (blockDim.x is a multiple of 64), so with yw the row made
+// uniform the plane and row offsets are scalar and a lane adds only a 32-bit
+// byte offset (host guarantees nx < 2^27).
+template <class U>
+__device__ __forceinline__ const U *at32(const U *base, int idx) {
+    return reinterpret_cast<const U *>(reinterpret_cast<const char *>(base) +
+                                       static_cast<unsigned>(idx) * static_cast<unsigned>(sizeof(U)));
+}
+template <class U>
+__device__ __forceinline__ U *at32(U *base, int idx) {
+    return reinterpret_cast<U *>(reinterpret_cast<char *>(base) +
+                                 static_cast<unsigned>(idx) * static_cast<unsigned>(sizeof(U)));
+}
+
+template <int QID, class ST, int VEC, int... Is>
+__device__ __forceinline__ void pull_interior_row(const typename ST::store_t *__restrict__ C,
+                                                  const StepParams<typename ST::T> &p, int yw,
+                                                  int x0, typename ST::T (&f)[VEC][QT<QID>::Q],
+                                                  std::integer_sequence<int, Is...>) {
+    using S = typename ST::store_t;
+    auto one = [&](auto Ic) {
+        constexpr int i = decltype(Ic)::value;
+        constexpr int cx = QT<QID>::cx(i), cy = QT<QID>::cy(i);
+        const long long row = (long long)p.nx * (yw - cy);
+        Pack<S, VEC> pk;
+        if constexpr (ST::kNeq) {
+            __builtin_memcpy(&pk, at32(ST::pops(C, p.plane) + p.plane * i + row, x0 - cx),
+                             sizeof(pk));
+            const Rec16 *m = at32(ST::recs(C) + row, x0 - cx);
+#pragma unroll
+            for (int v = 0; v < VEC; ++v)
+                f[v][i] = geq_rec<QID, i>(ST::unpack(m[v]), p) + ST::to_f(pk.v[v], p);
+        } else {
+            __builtin_memcpy(&pk, at32(C + p.plane * i + row, x0 - cx), sizeof(pk));
 #pragma unroll
             for (int v = 0; v < VEC; ++v) f[v][i] = ST::decode(pk.v[v], i, p);
         }
@@ -679,38 +799,38 @@ __device__ __forceinline__ void put_pack(S *dst, const Pack<S, VEC> &pk) {
     }
 }
 
-// NEQ storages: rec/geq are the collision's equilibrium record and values
-// (nullptr: derive them from f)
-template <int QID, class ST, int VEC, bool NT, int... Is>
+// Stores of VEC consecutive nodes starting at (x0, y).  ROW: wave-uniform row
+// addressing (see pull_interior_row); otherwise 64-bit node indices.  NEQ
+// storages store the collision's equilibrium records and the scaled
+// non-equilibrium parts nq.
+template <bool ROW, class U>
+__device__ __forceinline__ U *node_ptr(U *plane_base, int nx, int y, int x0) {
+    if constexpr (ROW) return at32(plane_base + (long long)nx * y, x0);
+    else return plane_base + ((long long)nx * y + x0);
+}
+
+template <int QID, class ST, int VEC, bool NT, bool ROW, int... Is>
 __device__ __forceinline__ void store_vec(typename ST::store_t *__restrict__ F,
-                                          const StepParams<typename ST::T> &p, long long node0,
+                                          const StepParams<typename ST::T> &p, int y, int x0,
                                           const typename ST::T (&f)[VEC][QT<QID>::Q],
                                           std::integer_sequence<int, Is...>,
-                                          const Mom4 *rec = nullptr,
-                                          const float (*geq)[QT<QID>::Q] = nullptr) {
+                                          const Rec16 *rec = nullptr,
+                                          const float (*nq)[QT<QID>::Q] = nullptr) {
     using S = typename ST::store_t;
     if constexpr (ST::kNeq) {
-        constexpr int Q = QT<QID>::Q;
-        float g[VEC][Q];
-        Mom4 *m = ST::mom(F) + node0;
+        Rec16 *m = node_ptr<ROW>(ST::recs(F), p.nx, y, x0);
 #pragma unroll
         for (int v = 0; v < VEC; ++v) {
-            Pack<Mom4, 1> pm;
-            if (rec) {
-                pm.v[0] = rec[v];
-#pragma unroll
-                for (int i = 0; i < Q; ++i) g[v][i] = f[v][i] - geq[v][i];
-            } else {
-                neq_split<QID>(f[v], p, pm.v[0], g[v]);
-            }
+            Pack<Rec16, 1> pm;
+            pm.v[0] = rec[v];
             put_pack<NT>(m + v, pm);
         }
         auto one = [&](auto Ic) {
             constexpr int i = decltype(Ic)::value;
             Pack<S, VEC> pk;
 #pragma unroll
-            for (int v = 0; v < VEC; ++v) pk.v[v] = ST::from_f(g[v][i], p);
-            put_pack<NT>(ST::pops(F, p.plane) + p.plane * i + node0, pk);
+            for (int v = 0; v < VEC; ++v) pk.v[v] = ST::raw(nq[v][i]);
+            put_pack<NT>(node_ptr<ROW>(ST::pops(F, p.plane) + p.plane * i, p.nx, y, x0), pk);
         };
         (one(std::integral_constant<int, Is>{}), ...);
     } else {
@@ -719,7 +839,7 @@ __device__ __forceinline__ void store_vec(typename ST::store_t *__restrict__ F,
             Pack<S, VEC> pk;
 #pragma unroll
             for (int v = 0; v < VEC; ++v) pk.v[v] = ST::encode(f[v][i], i, p);
-            put_pack<NT>(F + p.plane * i + node0, pk);
+            put_pack<NT>(node_ptr<ROW>(F + p.plane * i, p.nx, y, x0), pk);
         };
         (one(std::integral_constant<int, Is>{}), ...);
     }
@@ -731,29 +851,34 @@ k_fused_v(const typename ST::store_t *__restrict__ C, typename ST::store_t *__re
           const StepParams<typename ST::T> p) {
     constexpr int M = QT<QID>::MAXC;
     const int x0 = (blockIdx.x * blockDim.x + threadIdx.x) * VEC;
-    const int y = blockIdx.y * blockDim.y + threadIdx.y;  // blockDim.y rows per block
+    // blockDim.y rows per block; a wavefront never spans two rows (blockDim.x
+    // is a multiple of 64), so the row is wave-uniform
+    const int y = __builtin_amdgcn_readfirstlane(blockIdx.y * blockDim.y + threadIdx.y);
     if (x0 >= p.nx || y >= p.ny) return;
     typename ST::T f[VEC][QT<QID>::Q];
-    const long long node0 = (long long)x0 + (long long)p.nx * y;
+    const bool row = p.nx < (1 << 27);  // 32-bit byte offsets within a row
     const bool interior = (x0 >= M) && (x0 + VEC <= p.nx - M) && (y >= M) && (y < p.ny - M);
-    if (interior) {
-        pull_interior_vec<QID, ST, VEC>(C, p, node0, f, LBM_SEQ);
+    if (interior && row) {
+        pull_interior_row<QID, ST, VEC>(C, p, y, x0, f, LBM_SEQ);
+    } else if (interior) {
+        pull_interior_vec<QID, ST, VEC>(C, p, (long long)x0 + (long long)p.nx * y, f, LBM_SEQ);
     } else {
 #pragma unroll
         for (int v = 0; v < VEC; ++v) pull_seq<QID, ST>(C, p, x0 + v, y, f[v], LBM_SEQ);
     }
     if constexpr (ST::kNeq) {
-        // store f - feq with the collision's own equilibrium (no recomputation)
-        Mom4 rec[VEC];
-        float geq[VEC][QT<QID>::Q];
+        Rec16 rec[VEC];
+        float nq[VEC][QT<QID>::Q];
 #pragma unroll
         for (int v = 0; v < VEC; ++v)
-            collide_seq<QID, COLL, true, float>(f[v], p, x0 + v, y, LBM_SEQ, &rec[v], geq[v]);
-        store_vec<QID, ST, VEC, NT>(Cn, p, node0, f, LBM_SEQ, rec, geq);
+            collide_neq_seq<QID, COLL, ST>(f[v], p, x0 + v, y, rec[v], nq[v], LBM_SEQ);
+        if (row) store_vec<QID, ST, VEC, NT, true>(Cn, p, y, x0, f, LBM_SEQ, rec, nq);
+        else store_vec<QID, ST, VEC, NT, false>(Cn, p, y, x0, f, LBM_SEQ, rec, nq);
     } else {
 #pragma unroll
         for (int v = 0; v < VEC; ++v) collide_any<QID, COLL>(f[v], p, x0 + v, y);
-        store_vec<QID, ST, VEC, NT>(Cn, p, node0, f, LBM_SEQ);
+        if (row) store_vec<QID, ST, VEC, NT, true>(Cn, p, y, x0, f, LBM_SEQ);
+        else store_vec<QID, ST, VEC, NT, false>(Cn, p, y, x0, f, LBM_SEQ);
     }
 }
 
