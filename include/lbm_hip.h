@@ -113,7 +113,9 @@ int lbm_model_set_bcs(lbm_model m, const lbm_bc *bcs, int n);
 
 /* Launch shape of the fused step kernel: threads per block along x (64, 128,
  * 256), lattice rows per block (1, 2, 4; block_x * rows <= 256), lattice
- * nodes per thread along x (1, 2, 4, 8) and non-temporal stores (0/1);
+ * nodes per thread (1, 2, 4, 8; along x, except for the non-equilibrium
+ * storages on D2Q4/5/9, where they are rows marched by one thread: 1..16) and
+ * non-temporal stores (0/1);
  * 0 (or -1 for nontemporal) = automatic.  No counterpart in the reference (a
  * CPU loop has no launch shape). */
 int lbm_model_set_launch(lbm_model m, int block_x, int rows_per_block, int nodes_per_thread,

@@ -304,8 +304,15 @@ lbm::StepParams<double> make_params(const lbm_model_s *m, const lbm_field_s *f, 
     return p;
 }
 
+// non-equilibrium storages on one-speed stencils (D2Q4/5/9): nodes per thread
+// count lattice rows (k_fused_neq_rows), any nx
+bool rows_mode(int Q, int storage) {
+    return (storage == LBM_STORE_F16NEQ || storage == LBM_STORE_BF16NEQ) && Q <= 9;
+}
+
 // largest nodes-per-thread the kernels instantiate (mirrors max_vec in lbm_kernels.h)
 int max_vec(int Q, int storage) {
+    if (rows_mode(Q, storage)) return 16;
     const int eb = storage_bytes(storage);
     const int cap = eb >= 8 ? 2 : (eb == 4 ? 4 : 8);
     return Q <= 9 ? cap : (Q <= 17 ? (cap < 4 ? cap : 4) : 2);
@@ -313,10 +320,11 @@ int max_vec(int Q, int storage) {
 
 // automatic launch shape of the fused step.  Measured on MI355X, D2Q9 TRT
 // 4096^2: 256 x 1 blocks and non-temporal stores are fastest for every
-// storage; one node per thread for 8- and 4-byte populations and for the
-// non-equilibrium storages (more nodes per thread multiply the neighbour
-// moment records held in registers), four for the 16-bit deviation storages
-// (one dwordx2 per plane per lane).  lbm_autotune refines this per lattice.
+// storage; one node per thread for 8- and 4-byte populations, four for the
+// 16-bit deviation storages (one dwordx2 per plane per lane), and for the
+// non-equilibrium storages on one-speed stencils 8 rows per thread (the
+// records of the 3 x 3 neighbourhood move along in registers; 1 node per
+// thread on wider stencils).  lbm_autotune refines this per lattice.
 void auto_launch(const lbm_model_s *m, const lbm_field_s *f, int &block, int &rows, int &vec,
                  int &nt) {
     block = m->launch_block > 0 ? m->launch_block : 256;
@@ -324,6 +332,7 @@ void auto_launch(const lbm_model_s *m, const lbm_field_s *f, int &block, int &ro
     if (block * rows > 256) rows = 256 / block;
     vec = m->launch_vec;
     nt = m->launch_nt;
+    if (vec <= 0 && rows_mode(f->Q, f->storage) && m->coll != LBM_MRT) vec = 8;
     if (vec <= 0) {
         vec = 1;
         const bool neq = f->storage == LBM_STORE_F16NEQ || f->storage == LBM_STORE_BF16NEQ;
@@ -757,8 +766,8 @@ int lbm_model_set_launch(lbm_model m, int block_x, int rows_per_block, int nodes
     if (block_x * (rows_per_block > 0 ? rows_per_block : 1) > 256)
         return fail(-1, "block_x * rows_per_block must not exceed 256");
     if (nodes_per_thread != 0 && nodes_per_thread != 1 && nodes_per_thread != 2 &&
-        nodes_per_thread != 4 && nodes_per_thread != 8)
-        return fail(-1, "nodes_per_thread must be 0 (automatic), 1, 2, 4 or 8");
+        nodes_per_thread != 4 && nodes_per_thread != 8 && nodes_per_thread != 16)
+        return fail(-1, "nodes_per_thread must be 0 (automatic), 1, 2, 4, 8 or 16");
     m->launch_block = block_x;
     m->launch_rows = rows_per_block;
     m->launch_vec = nodes_per_thread;
@@ -790,11 +799,13 @@ int lbm_autotune(lbm_model m, lbm_field f_stream, int trial_steps, int *block_x,
     const int mv = (m->coll == LBM_MRT) ? 1 : max_vec(f_stream->Q, f_stream->storage);
     // (threads along x, rows) per block
     const int shapes[6][2] = {{64, 1}, {128, 1}, {256, 1}, {64, 4}, {128, 2}, {64, 2}};
+    const bool rows = rows_mode(f_stream->Q, f_stream->storage);
     for (int vec = 1; !rc && vec <= mv; vec <<= 1) {
-        if (f_stream->nx % vec) continue;
+        if (!rows && f_stream->nx % vec) continue;
         for (int nt = 0; !rc && nt <= (m->coll == LBM_MRT ? 0 : 1); ++nt) {
             for (int bi = 0; !rc && bi < 6; ++bi) {
                 if (m->coll == LBM_MRT && shapes[bi][1] != 1) continue;
+                if (rows && vec > 1 && shapes[bi][1] != 1) continue;  // one row per block
                 m->launch_block = shapes[bi][0];
                 m->launch_rows = shapes[bi][1];
                 m->launch_vec = vec;

@@ -671,6 +671,16 @@ __device__ __forceinline__ void collide_any(T (&f)[QT<QID>::Q], const StepParams
 // ---------------------------------------------------------------------------
 #define LBM_SEQ std::make_integer_sequence<int, QT<QID>::Q>{}
 
+// fn(std::integral_constant<int, i>) for every population i of QID
+template <int QID, class F, int... Is>
+__device__ __forceinline__ void each_node_seq(F &&fn, std::integer_sequence<int, Is...>) {
+    (fn(std::integral_constant<int, Is>{}), ...);
+}
+template <int QID, class F>
+__device__ __forceinline__ void adj_each_node(F &&fn) {
+    each_node_seq<QID>(fn, std::make_integer_sequence<int, QT<QID>::Q>{});
+}
+
 // collide!: f_out = collide(f_in), node-local
 template <int QID, int COLL, class ST>
 __global__ void __launch_bounds__(256)
@@ -882,6 +892,111 @@ k_fused_v(const typename ST::store_t *__restrict__ C, typename ST::store_t *__re
     }
 }
 
+// Non-equilibrium storages with a one-speed stencil (MAXC = 1): every thread
+// advances R consecutive rows of one lattice column.  The equilibrium
+// records of the three rows around the current one stay in LDS (packed, 8
+// bytes each): four rotating row slots of blockDim.x + 2 records (the block's
+// columns and one halo column on each side).  Every thread loads the record of its own
+// column for the row ahead and one barrier per row publishes it, so a node
+// loads one record instead of one per pulled population (9).  This kernel
+// updates the interior nodes; k_fused_edge updates the lattice edge (wrap +
+// boundary conditions).  Measured on MI355X (D2Q9 4096^2, FP16-NEQ): 8 rows,
+// 221 us per step, against 260 us with one node per thread (unpacking the
+// records to FP32 in LDS doubles the window and costs occupancy: 243 us).
+template <int QID, int COLL, class ST, int R, bool NT>
+__global__ void __launch_bounds__(256)
+k_fused_neq_rows(const typename ST::store_t *__restrict__ C,
+                     typename ST::store_t *__restrict__ Cn, const StepParams<float> p) {
+    static_assert(QT<QID>::MAXC == 1, "row marching keeps a 3-row record window");
+    constexpr int Q = QT<QID>::Q;
+    using S = typename ST::store_t;
+    __shared__ Rec16 win[4][256 + 2];
+    const int tid = threadIdx.x;
+    const int xb = blockIdx.x * blockDim.x;  // first column of the block
+    const int x = xb + tid;
+    const int y0 = __builtin_amdgcn_readfirstlane((int)blockIdx.y * R);
+    const int last = min((int)blockDim.x, p.nx - xb) - 1;  // last active thread
+    const bool active = tid <= last;
+    const Rec16 *rb = ST::recs(C);
+    auto load_row = [&](int yy) {
+        const int yw = yy < 0 ? yy + p.ny : (yy >= p.ny ? yy - p.ny : yy);
+        const Rec16 *r = rb + (long long)p.nx * yw;
+        Rec16 *w = win[yy & 3];
+        if (active) w[1 + tid] = *at32(r, x);
+        if (tid == 0) w[0] = *at32(r, xb == 0 ? p.nx - 1 : xb - 1);
+        if (tid == last) w[last + 2] = *at32(r, xb + last + 1 == p.nx ? 0 : xb + last + 1);
+    };
+    load_row(y0 - 1);
+    load_row(y0);
+    const bool xin = active && (x >= 1) && (x < p.nx - 1);
+#pragma unroll 1
+    for (int r = 0; r < R; ++r) {
+        const int y = y0 + r;
+        if (y >= p.ny) break;
+        load_row(y + 1);
+        __syncthreads();
+        if (xin && y >= 1 && y < p.ny - 1) {
+            float f[Q];
+            auto one = [&](auto Ic) {
+                constexpr int i = decltype(Ic)::value;
+                constexpr int cx = QT<QID>::cx(i), cy = QT<QID>::cy(i);
+                const S h = *at32(ST::pops(C, p.plane) + p.plane * i + (long long)p.nx * (y - cy),
+                                  x - cx);
+                f[i] = geq_rec<QID, i>(ST::unpack(win[(y - cy) & 3][1 + tid - cx]), p) +
+                       ST::to_f(h, p);
+            };
+            adj_each_node<QID>(one);
+            Rec16 rec;
+            float nq[Q];
+            collide_neq_seq<QID, COLL, ST>(f, p, x, y, rec, nq, LBM_SEQ);
+            Pack<Rec16, 1> pm;
+            pm.v[0] = rec;
+            put_pack<NT>(at32(ST::recs(Cn) + (long long)p.nx * y, x), pm);
+            auto st = [&](auto Ic) {
+                constexpr int i = decltype(Ic)::value;
+                Pack<S, 1> pk;
+                pk.v[0] = ST::raw(nq[i]);
+                put_pack<NT>(at32(ST::pops(Cn, p.plane) + p.plane * i + (long long)p.nx * y, x), pk);
+            };
+            adj_each_node<QID>(st);
+        }
+    }
+}
+
+// The lattice edge for k_fused_neq_rows: rows 0 and ny-1, then columns 0 and
+// nx-1 of the rows in between, one node per thread (general pull).
+template <int QID, int COLL, class ST>
+__global__ void __launch_bounds__(256)
+k_fused_edge(const typename ST::store_t *__restrict__ C, typename ST::store_t *__restrict__ Cn,
+             const StepParams<float> p) {
+    constexpr int Q = QT<QID>::Q;
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int nrow = p.ny > 1 ? 2 : 1, ncol = p.nx > 1 ? 2 : 1;
+    const long long top = (long long)nrow * p.nx;
+    int x, y;
+    if (k < top) {
+        y = k < p.nx ? 0 : p.ny - 1;
+        x = (int)(k < p.nx ? k : k - p.nx);
+    } else {
+        const long long j = k - top;
+        if (p.ny <= 2 || j >= (long long)ncol * (p.ny - 2)) return;
+        y = 1 + (int)(j % (p.ny - 2));
+        x = j < p.ny - 2 ? 0 : p.nx - 1;
+    }
+    float f[Q];
+    pull_seq<QID, ST>(C, p, x, y, f, LBM_SEQ);
+    Rec16 rec;
+    float nq[Q];
+    collide_neq_seq<QID, COLL, ST>(f, p, x, y, rec, nq, LBM_SEQ);
+    const long long n = (long long)x + (long long)p.nx * y;
+    ST::recs(Cn)[n] = rec;
+    auto st = [&](auto Ic) {
+        constexpr int i = decltype(Ic)::value;
+        ST::pops(Cn, p.plane)[p.plane * i + n] = ST::raw(nq[i]);
+    };
+    adj_each_node<QID>(st);
+}
+
 // largest nodes-per-thread worth instantiating (keeps VEC * Q registers bounded)
 template <int QID, class ST>
 constexpr int max_vec() {
@@ -990,10 +1105,40 @@ int launch_fused_vn(const Launch &L, const StepParams<typename ST::T> &p, hipStr
     return hipGetLastError() == hipSuccess ? 0 : -4;
 }
 
+// non-equilibrium storages, one-speed stencils: L.vec > 1 = rows per thread
+template <int QID, int COLL, class ST, int R>
+int launch_neq_rows_r(const Launch &L, const StepParams<typename ST::T> &p, hipStream_t s) {
+    using S = typename ST::store_t;
+    const int bx = L.block > 0 ? L.block : default_block(p.nx);
+    const dim3 grid((p.nx + bx - 1) / bx, (p.ny + R - 1) / R, 1), block(bx, 1, 1);
+    const S *src = static_cast<const S *>(L.src);
+    S *dst = static_cast<S *>(L.dst);
+    if (p.nx > 2 && p.ny > 2) {
+        if (L.nt) k_fused_neq_rows<QID, COLL, ST, R, true><<<grid, block, 0, s>>>(src, dst, p);
+        else k_fused_neq_rows<QID, COLL, ST, R, false><<<grid, block, 0, s>>>(src, dst, p);
+    }
+    const long long nedge = (long long)(p.ny > 1 ? 2 : 1) * p.nx +
+                            (p.ny > 2 ? (long long)(p.nx > 1 ? 2 : 1) * (p.ny - 2) : 0);
+    k_fused_edge<QID, COLL, ST><<<(unsigned)((nedge + 255) / 256), 256, 0, s>>>(src, dst, p);
+    return hipGetLastError() == hipSuccess ? 0 : -4;
+}
+
 template <int QID, int COLL, class ST>
 int launch_fused_c(const Launch &L, const StepParams<typename ST::T> &p, hipStream_t s) {
     constexpr int MV = max_vec<QID, ST>();
     const int vec = L.vec < 1 ? 1 : L.vec;
+    if constexpr (ST::kNeq && QT<QID>::MAXC == 1) {
+        if (vec > 1) {
+            if (p.nx >= (1 << 27)) return -5;
+            switch (vec) {
+            case 2: return launch_neq_rows_r<QID, COLL, ST, 2>(L, p, s);
+            case 4: return launch_neq_rows_r<QID, COLL, ST, 4>(L, p, s);
+            case 8: return launch_neq_rows_r<QID, COLL, ST, 8>(L, p, s);
+            case 16: return launch_neq_rows_r<QID, COLL, ST, 16>(L, p, s);
+            default: return -5;
+            }
+        }
+    }
     if (vec > MV || p.nx % vec != 0) return -5;
     switch (vec) {
     case 1: return launch_fused_vn<QID, COLL, ST, 1>(L, p, s);
