@@ -166,8 +166,8 @@ int lbm_batch_info(lbm_batch b, int *n, int64_t *nodes, int64_t *bytes);
  * f_stream, 1 = f_collision */
 int lbm_batch_upload(lbm_batch b, int k, const double *f);
 int lbm_batch_download(lbm_batch b, int k, int role, double *f);
-/* threads per tile-block (64, 128, 256) and persistent grid size (0 = one
- * resident wave of blocks per CU slot) */
+/* threads per tile-block (64, 128, 256; 0 = 128) and grid size (0 = one block
+ * per tile; smaller grids make the blocks stride over the tile table) */
 int lbm_batch_set_launch(lbm_batch b, int block, int grid);
 /* times launch shapes on the batch (state saved and restored), keeps the fastest */
 int lbm_batch_autotune(lbm_batch b, int trial_steps, int *block, int *grid, double *us_per_step);

@@ -962,7 +962,7 @@ struct lbm_batch_s {
     size_t pool_bytes = 0;
     int64_t nodes = 0;
     int parity_stream = 0;  // buffer holding f_stream
-    int block = 256, grid = 0;
+    int block = 128, grid = 0;
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
     int g_chunk = 0, g_parity = -1, g_block = 0, g_grid = 0;
@@ -988,14 +988,14 @@ int batch_tiles(lbm_batch b, int block) {
     return 0;
 }
 
+// default grid: one block per tile.  Measured on MI355X with the 512-lattice
+// batch (89.6M nodes, FP32): 128-thread tiles, one block each, 1.55 ms per
+// step; persistent grids of 1/4 to 4 resident waves of blocks were slower
+// (1.9-2.0 ms: every tile waits on its table reads in series).
 int batch_grid(const lbm_batch_s *b, int block) {
+    (void)block;
     if (b->grid > 0) return b->grid;
-    int dev = 0, cus = 256;
-    if (hipGetDevice(&dev) == hipSuccess)
-        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    const int per_cu = 2048 / block;  // one resident wave of blocks
-    const long long g = (long long)cus * per_cu;
-    return (int)(g < b->ntiles ? g : b->ntiles);
+    return b->ntiles;
 }
 
 lbm::StepParams<double> batch_params(const lbm_batch_s *b) {
@@ -1096,7 +1096,8 @@ int lbm_batch_create(lbm_batch *out, int quadrature, int storage, int n, const i
     const int eb = storage_bytes(storage);
     long long off = 0;
     for (int k = 0; k < n; ++k) {
-        if (nx[k] < 1 || ny[k] < 1 || (long long)nx[k] * ny[k] > (1LL << 30)) {
+        // node byte offsets within a lattice plane are 32-bit in the kernel
+        if (nx[k] < 1 || ny[k] < 1 || (long long)nx[k] * ny[k] > (1LL << 28)) {
             delete b;
             return fail(-1, "lattice %d: size %dx%d not supported", k, nx[k], ny[k]);
         }
@@ -1117,6 +1118,7 @@ int lbm_batch_create(lbm_batch *out, int quadrature, int storage, int n, const i
         L.om = 1.0 / ts;
         L.oms = 1.0 / ts;
         L.oma = 1.0 / ta;
+        L.inv_nx = 1.0 / (double)nx[k];
         L.coll = tau_a ? 1 : 0;
         b->lats.push_back(L);
         b->nx.push_back(nx[k]);
@@ -1210,7 +1212,7 @@ int lbm_batch_set_launch(lbm_batch b, int block, int grid) {
     if (block != 0 && block != 64 && block != 128 && block != 256)
         return fail(-1, "block must be 0 (automatic), 64, 128 or 256");
     if (grid < 0) return fail(-1, "grid must be >= 0");
-    b->block = block ? block : 256;
+    b->block = block ? block : 128;
     b->grid = grid;
     return 0;
 }
@@ -1272,8 +1274,14 @@ int lbm_batch_autotune(lbm_batch b, int trial_steps, int *block, int *grid, doub
     const int blocks[3] = {64, 128, 256};
     for (int bi = 0; !rc && bi < 3; ++bi) {
         const int per_cu = 2048 / blocks[bi];
-        const int grids[4] = {cus * per_cu / 4, cus * per_cu / 2, cus * per_cu, cus * per_cu * 4};
-        for (int gi = 0; !rc && gi < 4; ++gi) {
+        if (int r2 = batch_tiles(b, blocks[bi])) {
+            rc = r2;
+            break;
+        }
+        // persistent grids of 1/4 .. 4 resident waves of blocks, and one block per tile
+        const int grids[5] = {cus * per_cu / 4, cus * per_cu / 2, cus * per_cu, cus * per_cu * 4,
+                              b->ntiles};
+        for (int gi = 0; !rc && gi < 5; ++gi) {
             b->block = blocks[bi];
             b->grid = grids[gi];
             double ms = 0.0;

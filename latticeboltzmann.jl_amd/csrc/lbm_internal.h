@@ -88,6 +88,7 @@ struct BatchLattice {
     long long plane;
     long long off[2];  // ping-pong buffers
     double omo, om, oms, oma;  // SRT / TRT relaxation constants
+    double inv_nx;             // 1 / nx (row of a node index, corrected exactly)
     int coll;                  // 0 SRT, 1 TRT
     int pad;
 };

@@ -1195,8 +1195,9 @@ int launch_op(const Launch &L, const StepParams<typename ST::T> &p, hipStream_t 
 // batched independent periodic lattices: a persistent grid strides over a
 // table of tiles (blockDim.x consecutive nodes of one lattice each); lattice
 // geometry and relaxation times come from the lattice table (uniform per
-// tile, so they live in scalar registers).  OP: 0 collide, 1 fused step,
-// 2 stream.  All lattices share the ping-pong parity.
+// tile, so they live in scalar registers and the plane bases are scalar: a
+// lane adds a 32-bit offset).  OP: 0 collide, 1 fused step, 2 stream.  All
+// lattices share the ping-pong parity; stores are non-temporal.
 // ---------------------------------------------------------------------------
 template <int QID, class ST, int OP>
 __global__ void __launch_bounds__(256)
@@ -1205,6 +1206,8 @@ k_batch(const BatchLattice *__restrict__ lats, const BatchTile *__restrict__ til
         const StepParams<typename ST::T> base) {
     using T = typename ST::T;
     using S = typename ST::store_t;
+    constexpr int Q = QT<QID>::Q;
+    constexpr int M = QT<QID>::MAXC;
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const BatchTile tile = tiles[t];
         const BatchLattice L = lats[tile.lat];
@@ -1220,16 +1223,45 @@ k_batch(const BatchLattice *__restrict__ lats, const BatchTile *__restrict__ til
         S *dst = pool + L.off[parity ^ 1];
         const int n = tile.n0 + (int)threadIdx.x;
         if (n >= (int)L.plane) continue;
-        const int y = n / L.nx;
-        const int x = n - y * L.nx;
-        T f[QT<QID>::Q];
-        if constexpr (OP == 0) load_local<QID, ST>(src, p, n, f, LBM_SEQ);
-        else pull_seq<QID, ST>(src, p, x, y, f, LBM_SEQ);
+        int y = (int)((double)n * L.inv_nx);
+        int x = n - y * L.nx;
+        if (x < 0) {
+            --y;
+            x += L.nx;
+        } else if (x >= L.nx) {
+            ++y;
+            x -= L.nx;
+        }
+        T f[Q];
+        if constexpr (OP == 0) {
+            auto ld = [&](auto Ic) {
+                constexpr int i = decltype(Ic)::value;
+                f[i] = ST::decode(*at32(src + p.plane * i, n), i, p);
+            };
+            adj_each_node<QID>(ld);
+        } else {
+            if (x >= M && x < L.nx - M && y >= M && y < L.ny - M) {
+                auto ld = [&](auto Ic) {
+                    constexpr int i = decltype(Ic)::value;
+                    constexpr int cx = QT<QID>::cx(i), cy = QT<QID>::cy(i);
+                    f[i] = ST::decode(*at32(src + p.plane * i, n - cx - L.nx * cy), i, p);
+                };
+                adj_each_node<QID>(ld);
+            } else {
+                pull_seq<QID, ST>(src, p, x, y, f, LBM_SEQ);
+            }
+        }
         if constexpr (OP != 2) {
             if (L.coll == 0) collide_any<QID, 0>(f, p, x, y);
             else collide_any<QID, 1>(f, p, x, y);
         }
-        store_local<QID, ST>(dst, p, n, f, LBM_SEQ);
+        auto st = [&](auto Ic) {
+            constexpr int i = decltype(Ic)::value;
+            Pack<S, 1> pk;
+            pk.v[0] = ST::encode(f[i], i, p);
+            put_pack<true>(at32(dst + p.plane * i, n), pk);
+        };
+        adj_each_node<QID>(st);
     }
 }
 
