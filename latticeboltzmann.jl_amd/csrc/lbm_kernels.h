@@ -855,6 +855,74 @@ __device__ __forceinline__ void store_vec(typename ST::store_t *__restrict__ F,
     }
 }
 
+// Collision and store of one node, one population (SRT) or one opposite pair
+// (TRT) at a time, so that only the pulled populations stay live (not a
+// second array of equilibria): wide stencils (D2Q17, D2Q37) otherwise hold
+// 2 Q values per lane.  Same expressions as collide_seq (bit for bit).
+template <int QID, int COLL, class ST, bool NT, bool ROW, int... Is>
+__device__ __forceinline__ void collide_store_seq(const typename ST::T (&f)[QT<QID>::Q],
+                                                  const StepParams<typename ST::T> &p, int x,
+                                                  int y, typename ST::store_t *__restrict__ F,
+                                                  std::integer_sequence<int, Is...>) {
+    using T = typename ST::T;
+    using S = typename ST::store_t;
+    constexpr bool SHIFT = !std::is_same<T, double>::value;
+    T rho, ux, uy, drho = T(0);
+    if constexpr (SHIFT) {
+        node_moments_dev<QID, T>(f, p, drho, ux, uy);
+        rho = p.rho0 + drho;
+    } else {
+        node_moments<QID, T>(f, rho, ux, uy);
+    }
+    if (p.force_kind != 0) {
+        T Fx, Fy;
+        node_force(p, x, y, Fx, Fy);
+        ux = ux + p.tau_force * Fx;
+        uy = uy + p.tau_force * Fy;
+    }
+    const T usq = ux * ux + uy * uy;
+    Mom4 rec{};
+    if constexpr (SHIFT) rec = make_rec<QID>(drho, ux, uy, p);
+    auto feq = [&](auto Ic) -> T {
+        constexpr int i = decltype(Ic)::value;
+        if constexpr (SHIFT) return geq_rec<QID, i>(rec, p);
+        else return feq_i<QID, i, T>(rho, ux, uy, usq, p);
+    };
+    auto put = [&](auto Ic, T v) {
+        constexpr int i = decltype(Ic)::value;
+        Pack<S, 1> pk;
+        pk.v[0] = ST::encode(v, i, p);
+        put_pack<NT>(node_ptr<ROW>(F + p.plane * i, p.nx, y, x), pk);
+    };
+    auto one = [&](auto Ic) {
+        constexpr int i = decltype(Ic)::value;
+        constexpr int j = QT<QID>::opp(i);
+        if constexpr (COLL == 0) {
+            put(Ic, p.omo * f[i] + p.om * feq(Ic));  // srt.jl:57-59
+        } else if constexpr (i <= j) {
+            const T ei = feq(Ic), ej = feq(std::integral_constant<int, j>{});
+            if constexpr (SHIFT) {
+                const T ka = T(1) - T(0.5) * (p.oms + p.oma), kb = T(0.5) * (p.oma - p.oms);
+                const T ni = f[i] - ei, nj = f[j] - ej;
+                put(Ic, ei + (ka * ni + kb * nj));
+                if constexpr (j != i) put(std::integral_constant<int, j>{}, ej + (ka * nj + kb * ni));
+            } else {
+                // trt.jl:82-94 for i and for opp(i)
+                auto trt = [&](T gi, T gj, T qi, T qj) {
+                    const T feq_s = T(0.5) * (qi + qj);
+                    const T feq_a = T(0.5) * (qi - qj);
+                    const T f_s = T(0.5) * (gi + gj);
+                    const T f_a = T(0.5) * (gi - gj);
+                    return gi + ((-p.oms) * (f_s - feq_s) - p.oma * (f_a - feq_a));
+                };
+                put(Ic, trt(f[i], f[j], ei, ej));
+                if constexpr (j != i) put(std::integral_constant<int, j>{}, trt(f[j], f[i], ej, ei));
+            }
+        }
+    };
+    (one(std::integral_constant<int, Is>{}), ...);
+}
+
 template <int QID, int COLL, class ST, int VEC, bool NT>
 __global__ void __launch_bounds__(256)
 k_fused_v(const typename ST::store_t *__restrict__ C, typename ST::store_t *__restrict__ Cn,
@@ -884,6 +952,9 @@ k_fused_v(const typename ST::store_t *__restrict__ C, typename ST::store_t *__re
             collide_neq_seq<QID, COLL, ST>(f[v], p, x0 + v, y, rec[v], nq[v], LBM_SEQ);
         if (row) store_vec<QID, ST, VEC, NT, true>(Cn, p, y, x0, f, LBM_SEQ, rec, nq);
         else store_vec<QID, ST, VEC, NT, false>(Cn, p, y, x0, f, LBM_SEQ, rec, nq);
+    } else if constexpr (VEC == 1 && COLL != 2) {
+        if (row) collide_store_seq<QID, COLL, ST, NT, true>(f[0], p, x0, y, Cn, LBM_SEQ);
+        else collide_store_seq<QID, COLL, ST, NT, false>(f[0], p, x0, y, Cn, LBM_SEQ);
     } else {
 #pragma unroll
         for (int v = 0; v < VEC; ++v) collide_any<QID, COLL>(f[v], p, x0 + v, y);
