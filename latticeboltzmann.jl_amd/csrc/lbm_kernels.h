@@ -963,15 +963,56 @@ k_fused_v(const typename ST::store_t *__restrict__ C, typename ST::store_t *__re
     }
 }
 
+// Edge nodes of a one-speed lattice, enumerated: rows 0 and ny-1, then
+// columns 0 and nx-1 of the rows in between.
+__host__ __device__ inline long long edge_count(int nx, int ny) {
+    return (long long)(ny > 1 ? 2 : 1) * nx + (ny > 2 ? (long long)(nx > 1 ? 2 : 1) * (ny - 2) : 0);
+}
+__device__ inline bool edge_node(long long k, int nx, int ny, int &x, int &y) {
+    const int nrow = ny > 1 ? 2 : 1, ncol = nx > 1 ? 2 : 1;
+    const long long top = (long long)nrow * nx;
+    if (k < top) {
+        y = k < nx ? 0 : ny - 1;
+        x = (int)(k < nx ? k : k - nx);
+        return true;
+    }
+    const long long j = k - top;
+    if (ny <= 2 || j >= (long long)ncol * (ny - 2)) return false;
+    y = 1 + (int)(j % (ny - 2));
+    x = j < ny - 2 ? 0 : nx - 1;
+    return true;
+}
+
+// one edge node: general pull (wrap + boundary conditions), collide, store
+template <int QID, int COLL, class ST>
+__device__ __forceinline__ void neq_edge_update(const typename ST::store_t *__restrict__ C,
+                                                typename ST::store_t *__restrict__ Cn,
+                                                const StepParams<float> &p, int x, int y) {
+    constexpr int Q = QT<QID>::Q;
+    float f[Q];
+    pull_seq<QID, ST>(C, p, x, y, f, LBM_SEQ);
+    Rec16 rec;
+    float nq[Q];
+    collide_neq_seq<QID, COLL, ST>(f, p, x, y, rec, nq, LBM_SEQ);
+    const long long n = (long long)x + (long long)p.nx * y;
+    ST::recs(Cn)[n] = rec;
+    auto st = [&](auto Ic) {
+        constexpr int i = decltype(Ic)::value;
+        ST::pops(Cn, p.plane)[p.plane * i + n] = ST::raw(nq[i]);
+    };
+    adj_each_node<QID>(st);
+}
+
 // Non-equilibrium storages with a one-speed stencil (MAXC = 1): every thread
 // advances R consecutive rows of one lattice column.  The equilibrium
 // records of the three rows around the current one stay in LDS (packed, 8
 // bytes each): four rotating row slots of blockDim.x + 2 records (the block's
 // columns and one halo column on each side).  Every thread loads the record of its own
 // column for the row ahead and one barrier per row publishes it, so a node
-// loads one record instead of one per pulled population (9).  This kernel
-// updates the interior nodes; k_fused_edge updates the lattice edge (wrap +
-// boundary conditions).  Measured on MI355X (D2Q9 4096^2, FP16-NEQ): 8 rows,
+// loads one record instead of one per pulled population (9).  The blocks of
+// the last grid column update the lattice edge instead (general pull: wrap +
+// boundary conditions), a share per row of blocks, so the few edge nodes run
+// alongside the interior rather than as a separate latency-bound launch.  Measured on MI355X (D2Q9 4096^2, FP16-NEQ): 8 rows,
 // 221 us per step, against 260 us with one node per thread (unpacking the
 // records to FP32 in LDS doubles the window and costs occupancy: 243 us).
 template <int QID, int COLL, class ST, int R, bool NT>
@@ -983,6 +1024,17 @@ k_fused_neq_rows(const typename ST::store_t *__restrict__ C,
     using S = typename ST::store_t;
     __shared__ Rec16 win[4][256 + 2];
     const int tid = threadIdx.x;
+    if ((int)blockIdx.x == (int)gridDim.x - 1) {
+        // the last block column: this row of blocks' share of the edge nodes
+        const long long ne = edge_count(p.nx, p.ny);
+        const long long chunk = (ne + gridDim.y - 1) / gridDim.y;
+        const long long k1 = min(ne, (long long)(blockIdx.y + 1) * chunk);
+        for (long long k = (long long)blockIdx.y * chunk + tid; k < k1; k += blockDim.x) {
+            int ex, ey;
+            if (edge_node(k, p.nx, p.ny, ex, ey)) neq_edge_update<QID, COLL, ST>(C, Cn, p, ex, ey);
+        }
+        return;
+    }
     const int xb = blockIdx.x * blockDim.x;  // first column of the block
     const int x = xb + tid;
     const int y0 = __builtin_amdgcn_readfirstlane((int)blockIdx.y * R);
@@ -1034,38 +1086,15 @@ k_fused_neq_rows(const typename ST::store_t *__restrict__ C,
     }
 }
 
-// The lattice edge for k_fused_neq_rows: rows 0 and ny-1, then columns 0 and
-// nx-1 of the rows in between, one node per thread (general pull).
+// the edge nodes alone (lattices without interior nodes)
 template <int QID, int COLL, class ST>
 __global__ void __launch_bounds__(256)
 k_fused_edge(const typename ST::store_t *__restrict__ C, typename ST::store_t *__restrict__ Cn,
              const StepParams<float> p) {
-    constexpr int Q = QT<QID>::Q;
     const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const int nrow = p.ny > 1 ? 2 : 1, ncol = p.nx > 1 ? 2 : 1;
-    const long long top = (long long)nrow * p.nx;
     int x, y;
-    if (k < top) {
-        y = k < p.nx ? 0 : p.ny - 1;
-        x = (int)(k < p.nx ? k : k - p.nx);
-    } else {
-        const long long j = k - top;
-        if (p.ny <= 2 || j >= (long long)ncol * (p.ny - 2)) return;
-        y = 1 + (int)(j % (p.ny - 2));
-        x = j < p.ny - 2 ? 0 : p.nx - 1;
-    }
-    float f[Q];
-    pull_seq<QID, ST>(C, p, x, y, f, LBM_SEQ);
-    Rec16 rec;
-    float nq[Q];
-    collide_neq_seq<QID, COLL, ST>(f, p, x, y, rec, nq, LBM_SEQ);
-    const long long n = (long long)x + (long long)p.nx * y;
-    ST::recs(Cn)[n] = rec;
-    auto st = [&](auto Ic) {
-        constexpr int i = decltype(Ic)::value;
-        ST::pops(Cn, p.plane)[p.plane * i + n] = ST::raw(nq[i]);
-    };
-    adj_each_node<QID>(st);
+    if (!edge_node(k, p.nx, p.ny, x, y)) return;
+    neq_edge_update<QID, COLL, ST>(C, Cn, p, x, y);
 }
 
 // largest nodes-per-thread worth instantiating (keeps VEC * Q registers bounded)
@@ -1185,12 +1214,13 @@ int launch_neq_rows_r(const Launch &L, const StepParams<typename ST::T> &p, hipS
     const S *src = static_cast<const S *>(L.src);
     S *dst = static_cast<S *>(L.dst);
     if (p.nx > 2 && p.ny > 2) {
-        if (L.nt) k_fused_neq_rows<QID, COLL, ST, R, true><<<grid, block, 0, s>>>(src, dst, p);
-        else k_fused_neq_rows<QID, COLL, ST, R, false><<<grid, block, 0, s>>>(src, dst, p);
+        const dim3 g2(grid.x + 1, grid.y, 1);  // + one block column for the edge
+        if (L.nt) k_fused_neq_rows<QID, COLL, ST, R, true><<<g2, block, 0, s>>>(src, dst, p);
+        else k_fused_neq_rows<QID, COLL, ST, R, false><<<g2, block, 0, s>>>(src, dst, p);
+    } else {
+        const long long nedge = edge_count(p.nx, p.ny);
+        k_fused_edge<QID, COLL, ST><<<(unsigned)((nedge + 255) / 256), 256, 0, s>>>(src, dst, p);
     }
-    const long long nedge = (long long)(p.ny > 1 ? 2 : 1) * p.nx +
-                            (p.ny > 2 ? (long long)(p.nx > 1 ? 2 : 1) * (p.ny - 2) : 0);
-    k_fused_edge<QID, COLL, ST><<<(unsigned)((nedge + 255) / 256), 256, 0, s>>>(src, dst, p);
     return hipGetLastError() == hipSuccess ? 0 : -4;
 }
 

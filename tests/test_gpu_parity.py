@@ -444,16 +444,17 @@ def test_neq_row_marching_matches_single_node(storage, rows, shape, bcname):
     The two evaluate the same expressions in a different instruction order, so
     they agree bit for bit until an FP32 rounding difference flips one 16-bit
     rounding; from then on they differ by 16-bit ulps of the non-equilibrium
-    parts (~1e-5 w here) at a few nodes (measured: none after 1 step, under 2e-5 w
-    at under 5% of the values after 7).  A wrong record or edge mapping moves
-    whole neighbourhoods by w s du ~ 1e-2 w."""
+    parts (~1e-5 w here for binary16, ~3e-4 w for bfloat16) at a few nodes
+    (measured: under 5% of the values after 7 steps).  A wrong record or edge
+    mapping moves whole neighbourhoods by w s du ~ 1e-2 w."""
     L = _L()
     q = L.D2Q9()
     nx, ny = shape
     f0 = random_populations(9, nx, ny, seed=31, noise=1e-4)
     cm = L.TRT.fields(0.7, 1.3, force_for("const", nx, ny))
     bcs = to_lbmhip(bc_dicts(bcname, nx, ny))
-    for steps, tol, frac in ((1, 1e-4, 0.1), (7, 1e-4, 0.1)):
+    ulp_tol = {"f16neq": 1e-4, "bf16neq": 1e-3}[storage]  # bfloat16: 8x coarser than binary16
+    for steps, tol, frac in ((1, ulp_tol, 0.1), (7, ulp_tol, 0.1)):
         out = []
         # (block, rows per block, nodes per thread, NT): one node per thread,
         # row marching with 64- and 256-thread blocks
