@@ -365,7 +365,8 @@ def main():
                              "unit": "GB/s", "frac": round(gbs16 / HBM_PEAK_GBS, 4),
                              "traffic": traffic_for(f"d2q9_trt_{storage}_4096"),
                              "kernel": ("k_fused_neq_rows<9,TRT,F16Neq> (8 rows/thread, LDS "
-                                        "record window) + k_fused_edge" if storage == "f16neq"
+                                        "record window, edge nodes in its last block column)"
+                                        if storage == "f16neq"
                                         else "k_fused_v<9,TRT,F16Dev> (4 nodes/thread)"),
                              "kernel_ms_avg": round(kavg16, 5), "bytes_per_launch": bytes16},
             }
