@@ -84,6 +84,61 @@ def cpu_baseline(n, tau_s, tau_a, f0, target_s):
                       f"({dt:.1f} s) with the C restatement on {threads} OpenMP threads"}
 
 
+def processing_leg(L, q, model, n, u0, tau_s, t_steps, calls=20):
+    """One row of TrackHydrodynamicErrors on the FP64 4096^2 state, on the
+    device (lbm_track_errors: one pass over the populations and eight expected
+    planes, 11 numbers back) against the host path the reference takes
+    (download the populations, then the oracle's node loop on one core)."""
+    c = (np.arange(n) + 0.5) * (2 * np.pi / n)
+    X, Y = np.meshgrid(c, c, indexing="ij")
+    k = 2 * np.pi / n
+    nu = (tau_s - 0.5) / q.speed_of_sound_squared
+    # the vortex of tgv_populations in units of u0, its density = pressure, and
+    # -nu (grad u + grad u^T) / u0 as the stress
+    ux, uy = np.sin(X) * np.cos(Y), -np.cos(X) * np.sin(Y)
+    prho = -(3.0 * u0 * u0 / 4.0) * (np.cos(2 * X) + np.cos(2 * Y))
+    sxx = -nu * 2 * k * np.cos(X) * np.cos(Y) / u0
+    sxy = np.zeros_like(sxx)
+    planes = np.stack([prho, ux, uy, prho, sxx, sxy, sxy, -sxx], axis=-1)
+    d = float(np.exp(-2 * nu * k * k * t_steps))
+    affine = (1.0, d * d, 0.0, d, 0.0, d, 1.0, d * d, 0.0, d, 0.0, d, 0.0, d, 0.0, d)
+    ex = L.ExpectedFields(n, n, planes)
+    tau_p = q.speed_of_sound_squared * nu
+    row = L.track_errors(model.f_stream, ex, tau_p, u0, 1.0, affine)
+    L.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(calls):
+        row = L.track_errors(model.f_stream, ex, tau_p, u0, 1.0, affine)
+    dev_ms = (time.perf_counter() - t0) / calls * 1e3
+    per_node = q.Q * 8 + 8 * 8
+    out = {
+        "workload": f"one TrackHydrodynamicErrors row, D2Q9 FP64 {n}x{n} after {t_steps} steps",
+        "device_ms_per_row": round(dev_ms, 3),
+        "bytes_per_row": per_node * n * n,
+        "device_GBps_wall": round(per_node * n * n / (dev_ms * 1e-3) / 1e9, 1),
+        "error_rho": float(row[0]), "error_u": float(row[1]), "error_p": float(row[2]),
+        "mass": float(row[7]), "energy": float(row[9]),
+    }
+    from oracle import clib
+    t0 = time.perf_counter()
+    f = model.f_stream.to_numpy()
+    down_ms = (time.perf_counter() - t0) * 1e3
+    clib.set_threads(1)
+    t0 = time.perf_counter()
+    ref = clib.track_errors(9, f, np.asfortranarray(planes), affine, tau_p, u0, 1.0)
+    host_ms = (time.perf_counter() - t0) * 1e3
+    del f
+    out.update({
+        "host_path_ms": round(down_ms + host_ms, 1),
+        "host_path": "download the populations + the oracle's node loop on 1 core",
+        "download_ms": round(down_ms, 1),
+        "speedup_vs_host_path": round((down_ms + host_ms) / dev_ms, 1),
+        "max_rel_diff_vs_oracle": float(np.max(np.abs(row[:10] - ref[:10])
+                                               / np.maximum(np.abs(ref[:10]), 1e-300))),
+    })
+    return out
+
+
 def shear_wave_populations(L, q, n, amp, k):
     y = np.arange(n)
     ux = np.tile(amp * np.sin(2 * np.pi * k * y / n), (n, 1))
@@ -267,6 +322,7 @@ def main():
     ap.add_argument("--no-fp16", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-processing", action="store_true")
     ap.add_argument("--precision", action="store_true",
                     help="run only the 8192x2048 channel precision table (minutes)")
     ap.add_argument("--precision-steps", type=int, default=20000)
@@ -372,6 +428,9 @@ def main():
             }
             del m16
 
+    if not a.no_processing:
+        print("[bench] on-device processing row", file=sys.stderr, flush=True)
+        result["processing"] = processing_leg(L, q, m64, n, 0.02, cm.τ_symmetric, W + K)
     del m64
     if not a.no_batch:
         print("[bench] batch of 512 lattices", file=sys.stderr, flush=True)

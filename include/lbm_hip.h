@@ -214,6 +214,60 @@ int lbm_moments(lbm_field f, double tau, double *rho, double *ux, double *uy,
  * out[3] = sum rho |u|^2, out[4] = number of nodes with a non-finite population */
 int lbm_reduce(lbm_field f, double *out5);
 
+/* ---- on-device processing ----------------------------------------------- */
+/* Analytic reference fields for the error norms of TrackHydrodynamicErrors:
+ * eight Float64 (nx, ny) column-major planes, in this order:
+ * rho, u_x, u_y, p, sigma_xx, sigma_xy, sigma_yx, sigma_yy
+ * (the expected_* values of track_hydrodynamic_errors.jl:120-125).
+ * They stay on the device; a call combines them with per-call affine factors
+ * (lbm_track_params.affine), so fields that change in time only by a factor
+ * (a decaying vortex) are uploaded once. */
+typedef struct lbm_expected_s *lbm_expected;
+int lbm_expected_create(lbm_expected *out, int nx, int ny);
+int lbm_expected_destroy(lbm_expected e);
+/* planes: 8 * nx * ny doubles, plane k at planes + k * nx * ny */
+int lbm_expected_upload(lbm_expected e, const double *planes);
+
+#define LBM_TRACK_NOUT 11
+typedef struct {
+    /* relaxation time of the deviatoric stress: s * nu, without the 1/2
+     * (track_hydrodynamic_errors.jl:148-153, 176-177) */
+    double tau;
+    /* velocity scale: dimensionless u = u / u_max, sigma * (1 / u_max^2)
+     * (problems.jl:127-134) */
+    double u_max;
+    /* cell area multiplying mass, momentum and energy (Δ_ of
+     * track_hydrodynamic_errors.jl:85-94) */
+    double delta;
+    /* expected field k = affine[2k] + affine[2k+1] * plane k */
+    double affine[16];
+} lbm_track_params;
+
+/* One row of next!(::TrackHydrodynamicErrors) (track_hydrodynamic_errors.jl:
+ * 52-222) computed on the device: per node density, velocity!, the pressure
+ * tr(P)/2 and deviatoric_tensor (moments.jl:81-96), scaled to dimensionless
+ * units and compared with the expected fields, summed in FP64:
+ * out[0] error_rho, out[1] error_u, out[2] error_p, out[3] error_sigma_xx,
+ * out[4] error_sigma_xy, out[5] error_sigma_yy, out[6] error_sigma_yx,
+ * out[7] mass, out[8] momentum, out[9] energy (the reference's NamedTuple
+ * order, lines 205-221), out[10] number of nodes with a non-finite
+ * population.  NaNs propagate into the norms as in the reference. */
+int lbm_track_errors(lbm_field f, lbm_expected e, const lbm_track_params *prm, double *out);
+
+/* Previous velocity field of VelocityConvergenceStoppingCriteria
+ * (stopping_criteria.jl:57-69): two Float64 planes, zero at creation. */
+typedef struct lbm_velocity_tracker_s *lbm_velocity_tracker;
+int lbm_velocity_tracker_create(lbm_velocity_tracker *out, int nx, int ny);
+int lbm_velocity_tracker_destroy(lbm_velocity_tracker t);
+
+/* The sums behind the stopping criteria, in one pass over f:
+ * out[0] = mean u_x, out[1] = mean u_y (MeanVelocityStoppingCriteria,
+ * stopping_criteria.jl:17-39); with a tracker (else NULL):
+ * out[2] = sum |u - u_old|^2, out[3] = sum |u_old|^2, after which u_old = u
+ * (VelocityConvergenceStoppingCriteria, stopping_criteria.jl:71-99);
+ * out[4] = number of nodes with a non-finite population. */
+int lbm_velocity_stats(lbm_field f, lbm_velocity_tracker t, double *out5);
+
 #ifdef __cplusplus
 }
 #endif

@@ -188,6 +188,164 @@ k_reduce_final(const double *__restrict__ partials, int npart, double *__restric
         for (int k = 0; k < 5; ++k) out[k] = red[k][0];
 }
 
+// ---------------------------------------------------------------------------
+// On-device processing: one pass over f gives the sums behind a row of
+// next!(::TrackHydrodynamicErrors) (track_hydrodynamic_errors.jl:115-203) and
+// behind the stopping criteria (stopping_criteria.jl:17-99).  Each per-node
+// value is formed in the reference's order; the FP64 sums run over a fixed
+// grid-stride and tree order, so a result is reproducible bit for bit.
+template <int QID, int SID, bool ERR>
+__global__ void __launch_bounds__(256)
+k_track(const void *f, const StepParams<double> p, const TrackArgs a,
+        double *__restrict__ partials) {
+    constexpr int Q = QT<QID>::Q;
+    constexpr int NC = kTrackCols;
+    __shared__ double red[4][NC];
+    double acc[NC];
+#pragma unroll
+    for (int k = 0; k < NC; ++k) acc[k] = 0.0;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x; n < p.plane;
+         n += stride) {
+        double fl[Q];
+        bool finite = true;
+#pragma unroll
+        for (int i = 0; i < Q; ++i) {
+            fl[i] = load_d<SID>(f, p.plane, n, i, p);
+            finite = finite && isfinite(fl[i]);
+        }
+        // density and velocity! (moments.jl:1-19)
+        double rho = fl[0] + fl[1];
+#pragma unroll
+        for (int i = 2; i < Q; ++i) rho = rho + fl[i];
+        double ux = 0.0, uy = 0.0;
+#pragma unroll
+        for (int i = 0; i < Q; ++i) ux = ux + fl[i] * static_cast<double>(QT<QID>::cx(i));
+        ux = ux / rho;
+#pragma unroll
+        for (int i = 0; i < Q; ++i) uy = uy + fl[i] * static_cast<double>(QT<QID>::cy(i));
+        uy = uy / rho;
+        if (!finite) acc[kNonFinite] += 1.0;
+        // MeanVelocityStoppingCriteria: sum of u_x (stopping_criteria.jl:23-37)
+        acc[kSumUx] += ux;
+        acc[kSumUy] += uy;
+        // VelocityConvergenceStoppingCriteria (stopping_criteria.jl:81-99)
+        if (a.u_old) {
+            const double o1 = a.u_old[n], o2 = a.u_old[p.plane + n];
+            acc[kDu2] += (ux - o1) * (ux - o1) + (uy - o2) * (uy - o2);
+            acc[kUold2] += o1 * o1 + o2 * o2;
+            a.u_old[n] = ux;
+            a.u_old[p.plane + n] = uy;
+        }
+        if constexpr (ERR) {
+            // expected rho, u_x, u_y, p, sigma_xx, sigma_xy, sigma_yx, sigma_yy
+            double e[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) e[k] = a.aff[2 * k] + a.aff[2 * k + 1] * a.exp[p.plane * k + n];
+            // a_bar_2 = sum_i f_i H2_i, H2 = c c' - cs I (hermite_polynomials.jl:47-51);
+            // index k = alpha + 2 beta, column-major 2x2
+            const double cs = p.cs_inv;
+            double abar[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                auto h2 = [&](int i) -> double {
+                    const int al = k & 1, be = k >> 1;
+                    const int ca = al == 0 ? QT<QID>::cx(i) : QT<QID>::cy(i);
+                    const int cb = be == 0 ? QT<QID>::cx(i) : QT<QID>::cy(i);
+                    return static_cast<double>(cb * ca) - cs * static_cast<double>(al == be ? 1 : 0);
+                };
+                double s = fl[0] * h2(0) + fl[1] * h2(1);
+#pragma unroll
+                for (int i = 2; i < Q; ++i) s = s + fl[i] * h2(i);
+                abar[k] = s;
+            }
+            const double u[2] = {ux, uy};
+            // a_eq_2 = rho (u u' + cs (T - 1) I) at T = 1 (hermite.jl:45-48)
+            double aeq[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int al = k & 1, be = k >> 1;
+                aeq[k] = al == be ? rho * (u[al] * u[be] + 0.0) : rho * (u[al] * u[be]);
+            }
+            const double c = 1.0 / (2.0 * a.tau);
+            const double den = 1.0 + 1.0 / (2.0 * a.tau);
+            // a_2 = (a_bar_2 + a_eq_2 / (2 tau)) / (1 + 1/(2 tau)), P = a_2 - rho (u u' - I),
+            // p = tr(P) / D (track_hydrodynamic_errors.jl:153, 173, 181)
+            double P[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int al = k & 1, be = k >> 1;
+                const double a2 = (abar[k] + c * aeq[k]) / den;
+                P[k] = a2 - rho * (al == be ? u[al] * u[be] - 1.0 : u[al] * u[be]);
+            }
+            const double pr = ((0.0 + P[0]) + P[3]) / 2.0;
+            // sigma_lb = deviatoric_tensor(q, tau, f, rho, u) (moments.jl:81-96)
+            double sg[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) sg[k] = (abar[k] - aeq[k]) / den;
+            const double tr2 = ((0.0 + sg[0]) + sg[3]) / 2.0;
+            sg[0] = sg[0] - tr2;
+            sg[3] = sg[3] - tr2;
+            // dimensionless_velocity / _stress (problems.jl:127-134)
+            const double d1 = ux / a.u_max, d2 = uy / a.u_max;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) sg[k] = sg[k] * a.sfac;
+            // squared differences and norms with the cell weight 1
+            // (track_hydrodynamic_errors.jl:127-132, 192-202)
+            acc[kErrP] += (pr - e[3]) * (pr - e[3]);
+            acc[kErrRho] += (rho - e[0]) * (rho - e[0]);
+            acc[kErrU] += (d1 - e[1]) * (d1 - e[1]) + (d2 - e[2]) * (d2 - e[2]);
+            acc[kErrSxx] += (e[4] - sg[0]) * (e[4] - sg[0]);
+            acc[kErrSxy] += (e[5] - sg[2]) * (e[5] - sg[2]);
+            acc[kErrSyx] += (e[6] - sg[1]) * (e[6] - sg[1]);
+            acc[kErrSyy] += (e[7] - sg[3]) * (e[7] - sg[3]);
+            acc[kTotU2] += e[1] * e[1] + e[2] * e[2];
+            acc[kTotSxx2] += e[4] * e[4];
+            acc[kTotSyy2] += e[7] * e[7];
+            acc[kTotSxy2] += e[5] * e[5];
+            acc[kTotSyx2] += e[6] * e[6];
+            acc[kTotP2] += e[3] * e[3];
+            acc[kMass] += rho;
+            acc[kMomentum] += rho * (d1 + d2);
+            acc[kEnergy] += rho * (d1 * d1 + d2 * d2);
+        }
+    }
+    // wave sums by fixed shuffles, then the block's waves in order
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < NC; ++k) {
+        double v = acc[k];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+        if (lane == 0) red[wv][k] = v;
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < NC) {
+        const int nw = (int)blockDim.x >> 6;
+        double t = red[0][threadIdx.x];
+        for (int w = 1; w < nw; ++w) t += red[w][threadIdx.x];
+        partials[(long long)blockIdx.x * NC + threadIdx.x] = t;
+    }
+}
+
+// fixed-order sum of the partial rows, one column after another (one block)
+__global__ void __launch_bounds__(256)
+k_track_final(const double *__restrict__ partials, int npart, double *__restrict__ out) {
+    __shared__ double red[256];
+    for (int k = 0; k < kTrackCols; ++k) {
+        double v = 0.0;
+        for (int b = threadIdx.x; b < npart; b += blockDim.x) v += partials[(long long)b * kTrackCols + k];
+        red[threadIdx.x] = v;
+        __syncthreads();
+        for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+            if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) out[k] = red[0];
+        __syncthreads();
+    }
+}
+
 template <class U>
 __global__ void __launch_bounds__(256) k_copy_aux(const U *__restrict__ src, U *__restrict__ dst,
                                                   long long n) {
@@ -373,6 +531,45 @@ int dispatch_copy(const void *src, void *dst, size_t nbytes, hipStream_t s) {
         k_copy_aux<unsigned short><<<(int)blocks, 256, 0, s>>>(
             static_cast<const unsigned short *>(src), static_cast<unsigned short *>(dst), n);
     }
+    return hipGetLastError() == hipSuccess ? 0 : -4;
+}
+
+template <int QID>
+static int track_q(int storage, const void *f, const StepParams<double> &p, const TrackArgs &a,
+                   double *partials, int npart, hipStream_t s) {
+    const dim3 grid(npart, 1, 1), block(256, 1, 1);
+    const bool err = a.exp != nullptr;
+#define LBM_TRACK_CASE(SID)                                                            \
+    case SID:                                                                          \
+        if (err) k_track<QID, SID, true><<<grid, block, 0, s>>>(f, p, a, partials);   \
+        else k_track<QID, SID, false><<<grid, block, 0, s>>>(f, p, a, partials);      \
+        break;
+    switch (storage) {
+        LBM_TRACK_CASE(0)
+        LBM_TRACK_CASE(1)
+        LBM_TRACK_CASE(2)
+        LBM_TRACK_CASE(3)
+    default: return -1;
+    }
+#undef LBM_TRACK_CASE
+    return hipGetLastError() == hipSuccess ? 0 : -4;
+}
+
+int dispatch_track(int qid, int storage, const void *f, const StepParams<double> &p,
+                   const TrackArgs &a, double *partials, int npart, double *out, hipStream_t s) {
+    int rc;
+    switch (qid) {
+    case 4: rc = track_q<4>(storage, f, p, a, partials, npart, s); break;
+    case 5: rc = track_q<5>(storage, f, p, a, partials, npart, s); break;
+    case 9: rc = track_q<9>(storage, f, p, a, partials, npart, s); break;
+    case 13: rc = track_q<13>(storage, f, p, a, partials, npart, s); break;
+    case 17: rc = track_q<17>(storage, f, p, a, partials, npart, s); break;
+    case 21: rc = track_q<21>(storage, f, p, a, partials, npart, s); break;
+    case 37: rc = track_q<37>(storage, f, p, a, partials, npart, s); break;
+    default: return -1;
+    }
+    if (rc) return rc;
+    k_track_final<<<1, 256, 0, s>>>(partials, npart, out);
     return hipGetLastError() == hipSuccess ? 0 : -4;
 }
 

@@ -947,6 +947,165 @@ int lbm_reduce(lbm_field f, double *out5) {
     return 0;
 }
 
+// ---- on-device processing -----------------------------------------------------
+struct lbm_expected_s {
+    int nx = 0, ny = 0;
+    double *d = nullptr;  // 8 planes
+};
+
+struct lbm_velocity_tracker_s {
+    int nx = 0, ny = 0;
+    double *d = nullptr;  // u_x, u_y planes
+};
+
+static int planes_create(int nx, int ny, int nplanes, double **d, const char *what) {
+    if (nx < 1 || ny < 1) return fail(-1, "%s: lattice size %dx%d must be positive", what, nx, ny);
+    if ((long long)nx * (long long)ny > (1LL << 31) - 1)
+        return fail(-1, "%s: lattice of %lld nodes exceeds 2^31-1", what, (long long)nx * ny);
+    if (int rc = ensure_stream()) return rc;
+    const size_t nb = (size_t)nx * (size_t)ny * (size_t)nplanes * sizeof(double);
+    HIP_TRY(hipMalloc(d, nb));
+    hipError_t e = hipMemsetAsync(*d, 0, nb, g_stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(g_stream);
+    if (e != hipSuccess) {
+        (void)hipFree(*d);
+        *d = nullptr;
+        return fail(-3, "%s: clearing device planes failed: %s", what, hipGetErrorString(e));
+    }
+    return 0;
+}
+
+// one pass of k_track over f; sums[kTrackCols] on the host
+static int track_sums(lbm_field f, const lbm::TrackArgs &a, double *sums, const char *what) {
+    const lbm::StepParams<double> p = field_params(f);
+    const void *fdata;
+    int fstorage;
+    double *ftmp;
+    if (int rc = as_f64(f, &fdata, &fstorage, &ftmp)) return rc;
+    // fixed grid for a given lattice size: the summation order is reproducible
+    const long long blocks = (p.plane + 255) / 256;
+    const int npart = (int)(blocks < 2048 ? blocks : 2048);
+    double *part = nullptr, *out = nullptr;
+    hipError_t e = hipMalloc(&part, (size_t)npart * lbm::kTrackCols * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&out, lbm::kTrackCols * sizeof(double));
+    int rc = 0;
+    if (e == hipSuccess)
+        rc = lbm::dispatch_track(f->qid, fstorage, fdata, p, a, part, npart, out, g_stream);
+    if (e == hipSuccess && !rc)
+        e = hipMemcpyAsync(sums, out, lbm::kTrackCols * sizeof(double), hipMemcpyDeviceToHost,
+                           g_stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(g_stream);
+    free_dev(reinterpret_cast<void *&>(part));
+    free_dev(reinterpret_cast<void *&>(out));
+    free_dev(reinterpret_cast<void *&>(ftmp));
+    if (e != hipSuccess) return fail(-3, "%s: %s", what, hipGetErrorString(e));
+    if (rc) return fail(-4, "%s: processing kernel failed (%d)", what, rc);
+    return 0;
+}
+
+int lbm_expected_create(lbm_expected *out, int nx, int ny) {
+    if (!out) return fail(-1, "lbm_expected_create: null output handle");
+    *out = nullptr;
+    double *d = nullptr;
+    if (int rc = planes_create(nx, ny, 8, &d, "lbm_expected_create")) return rc;
+    lbm_expected e = new lbm_expected_s;
+    e->nx = nx;
+    e->ny = ny;
+    e->d = d;
+    *out = e;
+    return 0;
+}
+
+int lbm_expected_destroy(lbm_expected e) {
+    if (!e) return 0;
+    if (g_stream) (void)hipStreamSynchronize(g_stream);
+    free_dev(reinterpret_cast<void *&>(e->d));
+    delete e;
+    return 0;
+}
+
+int lbm_expected_upload(lbm_expected e, const double *planes) {
+    if (!e || !e->d) return fail(-1, "lbm_expected_upload: null handle");
+    if (!planes) return fail(-1, "lbm_expected_upload: null host planes");
+    const size_t nb = (size_t)e->nx * (size_t)e->ny * 8 * sizeof(double);
+    HIP_TRY(hipMemcpyAsync(e->d, planes, nb, hipMemcpyHostToDevice, g_stream));
+    HIP_TRY(hipStreamSynchronize(g_stream));
+    return 0;
+}
+
+int lbm_track_errors(lbm_field f, lbm_expected e, const lbm_track_params *prm, double *out) {
+    if (int rc = check_field(f, "lbm_track_errors")) return rc;
+    if (!e || !e->d) return fail(-1, "lbm_track_errors: null expected fields");
+    if (!prm || !out) return fail(-1, "lbm_track_errors: null parameters or output");
+    if (e->nx != f->nx || e->ny != f->ny)
+        return fail(-1, "lbm_track_errors: expected fields are %dx%d, the lattice %dx%d", e->nx,
+                    e->ny, f->nx, f->ny);
+    lbm::TrackArgs a{};
+    a.exp = e->d;
+    for (int k = 0; k < 16; ++k) a.aff[k] = prm->affine[k];
+    a.tau = prm->tau;
+    a.u_max = prm->u_max;
+    a.sfac = 1.0 / (prm->u_max * prm->u_max);
+    a.u_old = nullptr;
+    double s[lbm::kTrackCols];
+    if (int rc = track_sums(f, a, s, "lbm_track_errors")) return rc;
+    // the norms of the pushed row (track_hydrodynamic_errors.jl:205-221)
+    out[0] = std::sqrt(s[lbm::kErrRho]);
+    out[1] = std::sqrt(s[lbm::kErrU] / s[lbm::kTotU2]);
+    out[2] = std::sqrt(s[lbm::kErrP] / s[lbm::kTotP2]);
+    out[3] = std::sqrt(s[lbm::kErrSxx] / s[lbm::kTotSxx2]);
+    out[4] = std::sqrt(s[lbm::kErrSxy] / s[lbm::kTotSxy2]);
+    out[5] = std::sqrt(s[lbm::kErrSyy] / s[lbm::kTotSyy2]);
+    out[6] = std::sqrt(s[lbm::kErrSyx] / s[lbm::kTotSyx2]);
+    out[7] = prm->delta * s[lbm::kMass];
+    out[8] = prm->delta * s[lbm::kMomentum];
+    out[9] = prm->delta * s[lbm::kEnergy];
+    out[10] = s[lbm::kNonFinite];
+    return 0;
+}
+
+int lbm_velocity_tracker_create(lbm_velocity_tracker *out, int nx, int ny) {
+    if (!out) return fail(-1, "lbm_velocity_tracker_create: null output handle");
+    *out = nullptr;
+    double *d = nullptr;
+    if (int rc = planes_create(nx, ny, 2, &d, "lbm_velocity_tracker_create")) return rc;
+    lbm_velocity_tracker t = new lbm_velocity_tracker_s;
+    t->nx = nx;
+    t->ny = ny;
+    t->d = d;
+    *out = t;
+    return 0;
+}
+
+int lbm_velocity_tracker_destroy(lbm_velocity_tracker t) {
+    if (!t) return 0;
+    if (g_stream) (void)hipStreamSynchronize(g_stream);
+    free_dev(reinterpret_cast<void *&>(t->d));
+    delete t;
+    return 0;
+}
+
+int lbm_velocity_stats(lbm_field f, lbm_velocity_tracker t, double *out5) {
+    if (int rc = check_field(f, "lbm_velocity_stats")) return rc;
+    if (!out5) return fail(-1, "lbm_velocity_stats: null output");
+    if (t && (t->nx != f->nx || t->ny != f->ny))
+        return fail(-1, "lbm_velocity_stats: tracker is %dx%d, the lattice %dx%d", t->nx, t->ny,
+                    f->nx, f->ny);
+    lbm::TrackArgs a{};
+    a.exp = nullptr;
+    a.u_old = t ? t->d : nullptr;
+    double s[lbm::kTrackCols];
+    if (int rc = track_sums(f, a, s, "lbm_velocity_stats")) return rc;
+    // u_mean /= divide_by (stopping_criteria.jl:39), divide_by = nx * ny
+    const double nodes = (double)f->nx * (double)f->ny;
+    out5[0] = s[lbm::kSumUx] / nodes;
+    out5[1] = s[lbm::kSumUy] / nodes;
+    out5[2] = t ? s[lbm::kDu2] : 0.0;
+    out5[3] = t ? s[lbm::kUold2] : 0.0;
+    out5[4] = s[lbm::kNonFinite];
+    return 0;
+}
+
 }  // extern "C"
 
 // ---------------------------------------------------------------------------

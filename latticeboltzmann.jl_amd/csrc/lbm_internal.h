@@ -163,6 +163,24 @@ int dispatch_moments(int qid, int storage, const void *f, const StepParams<doubl
                      hipStream_t s);
 int dispatch_reduce(int qid, int storage, const void *f, const StepParams<double> &p,
                     double *partials, int npart, double *out5, hipStream_t s);
+// on-device processing: TrackHydrodynamicErrors norms and stopping-criteria sums
+constexpr int kTrackCols = 21;
+struct TrackArgs {
+    const double *exp;  // 8 expected planes (nx*ny each) or nullptr: velocity sums only
+    double aff[16];     // expected_k = aff[2k] + aff[2k+1] * plane_k
+    double tau;         // sigma relaxation time (s * nu)
+    double u_max;       // dimensionless velocity u / u_max
+    double sfac;        // dimensionless stress factor 1 / u_max^2
+    double *u_old;      // previous velocity, 2 planes, updated in place; or nullptr
+};
+// columns of the sums (kTrackCols of them), in this order
+enum TrackCol : int {
+    kErrRho = 0, kErrU, kErrP, kErrSxx, kErrSxy, kErrSyy, kErrSyx,
+    kTotU2, kTotP2, kTotSxx2, kTotSxy2, kTotSyy2, kTotSyx2,
+    kMass, kMomentum, kEnergy, kSumUx, kSumUy, kDu2, kUold2, kNonFinite
+};
+int dispatch_track(int qid, int storage, const void *f, const StepParams<double> &p,
+                   const TrackArgs &a, double *partials, int npart, double *out, hipStream_t s);
 int dispatch_upload(int qid, int storage, const double *src, void *dst,
                     const StepParams<double> &p, int to_device, hipStream_t s);
 int dispatch_copy(const void *src, void *dst, size_t nbytes, hipStream_t s);

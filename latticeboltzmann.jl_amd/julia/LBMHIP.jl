@@ -225,11 +225,126 @@ function simulate(model::LatticeBoltzmannModel{<:Quadrature, <:HipLattice}, time
                     m.handle, model.f_stream.handle, model.f_collision.handle, n),
               "lbm_simulate")
         t = stop + 1
-        if LatticeBoltzmann.next!(pm, model.quadrature, Array(model.f_stream), t)
+        # TrackHydrodynamicErrors reads the device array (methods below); any
+        # other processing method gets a host copy
+        f = pm isa LatticeBoltzmann.TrackHydrodynamicErrors ? model.f_stream :
+            Array(model.f_stream)
+        if LatticeBoltzmann.next!(pm, model.quadrature, f, t)
             return model
         end
     end
     model
+end
+
+# ---- on-device processing ---------------------------------------------------
+# next!(::TrackHydrodynamicErrors) (track_hydrodynamic_errors.jl:52-222) and
+# should_stop! (stopping_criteria.jl:17-115) on a HipLattice: one kernel pass
+# each (lbm_track_errors, lbm_velocity_stats); the populations stay on the GPU.
+struct CTrackParams
+    tau::Cdouble
+    u_max::Cdouble
+    delta::Cdouble
+    affine::NTuple{16, Cdouble}
+end
+
+mutable struct ExpectedFields
+    handle::Ptr{Cvoid}
+end
+
+function ExpectedFields(planes::Array{Float64, 3})
+    nx, ny, k = size(planes)
+    k == 8 || throw(ArgumentError("8 expected planes needed, got $k"))
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:lbm_expected_create, libpath), Cint, (Ptr{Ptr{Cvoid}}, Cint, Cint), h, nx, ny),
+          "lbm_expected_create")
+    e = finalizer(ExpectedFields(h[])) do x
+        ccall((:lbm_expected_destroy, libpath), Cint, (Ptr{Cvoid},), x.handle)
+    end
+    GC.@preserve planes check(ccall((:lbm_expected_upload, libpath), Cint,
+                                    (Ptr{Cvoid}, Ptr{Cdouble}), e.handle, planes),
+                              "lbm_expected_upload")
+    e
+end
+
+# the analytic fields of the problem at `time`, in the planes' order
+function expected_planes(q, problem, time)
+    x_range, y_range = LatticeBoltzmann.range(problem)
+    P = Array{Float64}(undef, length(x_range), length(y_range), 8)
+    for x_idx in eachindex(x_range), y_idx in eachindex(y_range)
+        x, y = x_range[x_idx], y_range[y_idx]
+        u = LatticeBoltzmann.velocity(problem, x, y, time)
+        σ = LatticeBoltzmann.deviatoric_tensor(q, problem, x, y, time)
+        P[x_idx, y_idx, :] .= (LatticeBoltzmann.density(q, problem, x, y, time), u[1], u[2],
+                               LatticeBoltzmann.pressure(q, problem, x, y, time),
+                               σ[1, 1], σ[1, 2], σ[2, 1], σ[2, 2])
+    end
+    P
+end
+
+function cell_area(problem)
+    x_range, y_range = LatticeBoltzmann.range(problem)
+    nx, ny = length(x_range), length(y_range)
+    nx == 1 && return ny == 1 ? 1.0 : Float64(y_range.step)
+    ny == 1 && return Float64(x_range.step)
+    Float64(y_range.step) * Float64(x_range.step)
+end
+
+function LatticeBoltzmann.next!(pm::LatticeBoltzmann.TrackHydrodynamicErrors, q,
+                                f::HipLattice, t::Int64)
+    should_stop = false
+    if mod(t, 100) == 0 && LatticeBoltzmann.should_stop!(pm.stop_criteria, q, f)
+        @info "Stopping after $t iterations"
+        should_stop = true
+    end
+    (should_stop == false && t != pm.n_steps && !pm.should_process) && return false
+    problem = pm.problem
+    e = ExpectedFields(expected_planes(q, problem, t * LatticeBoltzmann.delta_t(problem)))
+    prm = Ref(CTrackParams(q.speed_of_sound_squared * LatticeBoltzmann.lattice_viscosity(problem),
+                           problem.u_max, cell_area(problem),
+                           ntuple(k -> isodd(k) ? 0.0 : 1.0, 16)))
+    out = zeros(11)
+    GC.@preserve out check(ccall((:lbm_track_errors, libpath), Cint,
+                                 (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{CTrackParams}, Ptr{Cdouble}),
+                                 f.handle, e.handle, prm, out), "lbm_track_errors")
+    push!(pm.df, (timestep = t, error_ρ = out[1], error_u = out[2], error_p = out[3],
+                  error_σ_xx = out[4], error_σ_xy = out[5], error_σ_yy = out[6],
+                  error_σ_yx = out[7], mass = out[8], momentum = out[9], energy = out[10]))
+    should_stop
+end
+
+function velocity_stats(f::HipLattice, tracker::Ptr{Cvoid} = C_NULL)
+    out = zeros(5)
+    GC.@preserve out check(ccall((:lbm_velocity_stats, libpath), Cint,
+                                 (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cdouble}), f.handle, tracker, out),
+                           "lbm_velocity_stats")
+    out
+end
+
+function LatticeBoltzmann.should_stop!(c::LatticeBoltzmann.MeanVelocityStoppingCriteria, q,
+                                       f::HipLattice)
+    u_mean = velocity_stats(f)[1]
+    abs(u_mean / c.old_mean_velocity - 1) < c.tolerance && return true
+    isnan(u_mean) && (@warn "nan in velocity profile"; return true)
+    c.old_mean_velocity = u_mean
+    false
+end
+
+# the previous velocity of each VelocityConvergenceStoppingCriteria, on the device
+const TRACKERS = IdDict{Any, Ptr{Cvoid}}()
+
+function LatticeBoltzmann.should_stop!(c::LatticeBoltzmann.VelocityConvergenceStoppingCriteria,
+                                       q, f::HipLattice)
+    tr = get!(TRACKERS, c) do
+        h = Ref{Ptr{Cvoid}}(C_NULL)
+        check(ccall((:lbm_velocity_tracker_create, libpath), Cint, (Ptr{Ptr{Cvoid}}, Cint, Cint),
+                    h, f.dims[1], f.dims[2]), "lbm_velocity_tracker_create")
+        h[]
+    end
+    s = velocity_stats(f, tr)
+    converged = sqrt(s[3]) / s[4]
+    converged < c.tolerance && return true
+    isnan(converged) && (@warn "nan in velocity profile"; return true)
+    false
 end
 
 # ---- reverse mode -----------------------------------------------------------

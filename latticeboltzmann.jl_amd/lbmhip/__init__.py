@@ -17,6 +17,11 @@ from .lattice import HipLattice  # noqa: F401
 from .model import (LatticeBoltzmannModel, ProcessingMethod, apply_boundary_conditions_,  # noqa: F401
                     autotune, collide_model_, next_, simulate, simulate_timed, stream_model_)
 from .moments import conserved_sums, density, moments, velocity  # noqa: F401
+from .problems import TaylorGreenVortex  # noqa: F401
+from .processing import (DensityConvergence, ExpectedFields, MeanVelocityStoppingCriteria,  # noqa: F401
+                         NoStoppingCriteria, StopCriteria, TrackHydrodynamicErrors, TrackRow,
+                         VelocityConvergenceStoppingCriteria, VelocityTracker,
+                         stop_criteria_for, track_errors, velocity_stats)
 from .operators import apply_, collide_, stream_, stream_bc_, synchronize  # noqa: F401
 from .quadratures import (D2Q4, D2Q5, D2Q9, D2Q13, D2Q17, D2Q21, D2Q37, Quadrature,  # noqa: F401
                           Quadratures, dimension, opposite, order)

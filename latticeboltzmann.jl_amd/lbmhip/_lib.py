@@ -35,6 +35,8 @@ EXPORTS = [
     "lbm_batch_download", "lbm_batch_set_launch", "lbm_batch_autotune", "lbm_batch_simulate",
     "lbm_adjoint_create", "lbm_adjoint_destroy", "lbm_adjoint_info", "lbm_adjoint_loss",
     "lbm_adjoint_gradient",
+    "lbm_expected_create", "lbm_expected_destroy", "lbm_expected_upload", "lbm_track_errors",
+    "lbm_velocity_tracker_create", "lbm_velocity_tracker_destroy", "lbm_velocity_stats",
 ]
 
 
@@ -49,6 +51,15 @@ class LbmBC(ctypes.Structure):
         ("ux", ctypes.c_double),
         ("uy", ctypes.c_double),
         ("rho", ctypes.c_double),
+    ]
+
+
+class LbmTrackParams(ctypes.Structure):
+    _fields_ = [
+        ("tau", ctypes.c_double),
+        ("u_max", ctypes.c_double),
+        ("delta", ctypes.c_double),
+        ("affine", ctypes.c_double * 16),
     ]
 
 
@@ -114,6 +125,13 @@ def lib() -> ctypes.CDLL:
     L.lbm_adjoint_info.argtypes = [vp, ip, ip, ctypes.POINTER(i64)]
     L.lbm_adjoint_loss.argtypes = [vp, dp, dp, dp, dp]
     L.lbm_adjoint_gradient.argtypes = [vp, dp, dp, dp, dp, dp, dp]
+    L.lbm_expected_create.argtypes = [ctypes.POINTER(vp), i, i]
+    L.lbm_expected_destroy.argtypes = [vp]
+    L.lbm_expected_upload.argtypes = [vp, dp]
+    L.lbm_track_errors.argtypes = [vp, vp, ctypes.POINTER(LbmTrackParams), dp]
+    L.lbm_velocity_tracker_create.argtypes = [ctypes.POINTER(vp), i, i]
+    L.lbm_velocity_tracker_destroy.argtypes = [vp]
+    L.lbm_velocity_stats.argtypes = [vp, vp, dp]
     for name in EXPORTS:
         f = getattr(L, name)
         if name not in ("lbm_last_error",):

@@ -33,7 +33,9 @@ from .quadratures import Quadrature
 
 class ProcessingMethod:
     """Per-step hook: ``next_(q, f_stream, t) -> bool`` (True stops the run);
-    processing_methods.jl:92."""
+    processing_methods.jl:92.  A method that defines ``needs_step(t)`` promises
+    that ``next_`` at the other steps returns False and changes nothing; the
+    steps in between then run as fused launches."""
 
     def next_(self, q: Quadrature, f: HipLattice, t: int) -> bool:
         return False
@@ -103,13 +105,30 @@ def simulate(model: LatticeBoltzmannModel, time) -> LatticeBoltzmannModel:
         next_(model, 0)
         return model
     force = getattr(model.collision_model, "force", None)
-    if model.processing_method is None and not callable(force):
+    pm = model.processing_method
+    needs = getattr(pm, "needs_step", None) if pm is not None else None
+    if not callable(force) and (pm is None or callable(needs)):
+        # fused steps, broken only where the processing method reads the state
         m = native_model(model.quadrature).configure(model.collision_model, model.f_stream,
                                                      model.boundary_conditions,
                                                      float(steps[0]) * model.delta_t)
         m.set_launch(model.launch)
-        _lib.check(_lib.lib().lbm_simulate(m.handle, model.f_stream.handle,
-                                           model.f_collision.handle, len(steps)), "simulate")
+        n = len(steps)
+        i = 0
+        while i < n:
+            j = i
+            while pm is not None and j < n and not needs(int(steps[j]) + 1):
+                j += 1
+            if pm is None:
+                j = n
+            count = min(j, n - 1) - i + 1
+            _lib.check(_lib.lib().lbm_simulate(m.handle, model.f_stream.handle,
+                                               model.f_collision.handle, count), "simulate")
+            if j < n and next_(model, int(steps[j]) + 1):
+                return model
+            i += count
+        if pm is not None:
+            next_(model, int(steps[-1]) + 1)
         return model
     m = native_model(model.quadrature)
     for t in steps:

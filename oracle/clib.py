@@ -86,6 +86,8 @@ def lib() -> ctypes.CDLL:
         _lib.lbo_simulate.restype = i
         _lib.lbo_simulate.argtypes = [qp, i, i, dp, dp, i, dp, i, i, dp, bp, i, ctypes.c_long]
         _lib.lbo_moments.argtypes = [qp, i, i, dp, d, dp, dp, dp, dp, dp, dp]
+        _lib.lbo_track_errors.argtypes = [qp, i, i, dp, dp, dp, d, d, d, dp]
+        _lib.lbo_velocity_stats.argtypes = [qp, i, i, dp, dp, dp]
         _lib.lbo_set_threads.argtypes = [i]
         _lib.lbo_get_threads.restype = i
     return _lib
@@ -249,3 +251,37 @@ def moments(qid, f, tau):
     outs = [np.zeros((nx, ny), order="F") for _ in range(6)]
     lib().lbo_moments(ctypes.byref(q), nx, ny, _dp(f), float(tau), *[_dp(o) for o in outs])
     return outs
+
+
+def track_errors(qid, f, expected, affine, tau, u_max, delta=1.0):
+    """One TrackHydrodynamicErrors row (track_hydrodynamic_errors.jl:52-222):
+    error_rho, error_u, error_p, error_sxx, error_sxy, error_syy, error_syx,
+    mass, momentum, energy, non-finite nodes.  expected: (nx, ny, 8), planes
+    rho, ux, uy, p, sxx, sxy, syx, syy."""
+    q = quadrature(qid)
+    _check_f(f, q.Q)
+    nx, ny, _ = f.shape
+    e = np.asfortranarray(expected, dtype=np.float64)
+    assert e.shape == (nx, ny, 8)
+    a = np.ascontiguousarray(affine, dtype=np.float64)
+    assert a.size == 16
+    out = np.zeros(11)
+    lib().lbo_track_errors(ctypes.byref(q), nx, ny, _dp(f), _dp(e), _dp(a), float(tau),
+                           float(u_max), float(delta), _dp(out))
+    return out
+
+
+def velocity_stats(qid, f, u_old=None):
+    """Stopping-criteria sums (stopping_criteria.jl:17-99): mean ux, mean uy,
+    sum |u - u_old|^2, sum |u_old|^2, non-finite nodes.  u_old: (nx, ny, 2)
+    Fortran array, updated in place to u."""
+    q = quadrature(qid)
+    _check_f(f, q.Q)
+    nx, ny, _ = f.shape
+    if u_old is not None:
+        assert u_old.shape == (nx, ny, 2) and u_old.flags.f_contiguous
+        assert u_old.dtype == np.float64
+    out = np.zeros(5)
+    lib().lbo_velocity_stats(ctypes.byref(q), nx, ny, _dp(f),
+                             None if u_old is None else _dp(u_old), _dp(out))
+    return out

@@ -313,3 +313,53 @@ def deviatoric_field(q, f, tau):
     sig = a / (1.0 + 1.0 / (2.0 * tau))
     tr = 0.5 * (sig[..., 0, 0] + sig[..., 1, 1])
     return sig[..., 0, 0] - tr, sig[..., 0, 1], sig[..., 1, 1] - tr
+
+
+def track_errors(q, f, expected, affine, tau, u_max, delta=1.0):
+    """One TrackHydrodynamicErrors row (track_hydrodynamic_errors.jl:52-222) as
+    whole-lattice tensor algebra: the second Hermite moment, the pressure as
+    the trace of P = a_2 - rho (u u - I), sigma from the deviatoric part.
+    expected: (nx, ny, 8) planes rho, ux, uy, p, sxx, sxy, syx, syy."""
+    a = np.asarray(affine, dtype=np.float64).reshape(8, 2)
+    e = a[:, 0] + a[:, 1] * np.asarray(expected, dtype=np.float64)
+    cs = 1.0 / q.s
+    rho, ux, uy = moments_field(q, f)
+    cc = q.c.astype(float)
+    abar = np.einsum("xyi,ai,bi->xyab", f, cc, cc) - cs * rho[..., None, None] * np.eye(2)
+    u = np.stack([ux, uy], axis=-1)
+    uu = np.einsum("xya,xyb->xyab", u, u)
+    aeq = rho[..., None, None] * uu
+    k = 1.0 / (2.0 * tau)
+    a2 = (abar + k * aeq) / (1.0 + k)
+    P = a2 - rho[..., None, None] * (uu - np.eye(2))
+    p = np.trace(P, axis1=2, axis2=3) / 2.0
+    sig = (abar - aeq) / (1.0 + k)
+    sig = sig - np.eye(2) * (np.trace(sig, axis1=2, axis2=3) / 2.0)[..., None, None]
+    sig = sig / u_max ** 2
+    d = u / u_max
+    ex_s = {(0, 0): e[..., 4], (0, 1): e[..., 5], (1, 0): e[..., 6], (1, 1): e[..., 7]}
+
+    def rel(err, ref):
+        return math.sqrt(np.sum(err ** 2) / np.sum(ref ** 2))
+
+    out = [math.sqrt(np.sum((rho - e[..., 0]) ** 2)),
+           math.sqrt(np.sum((d[..., 0] - e[..., 1]) ** 2 + (d[..., 1] - e[..., 2]) ** 2)
+                     / np.sum(e[..., 1] ** 2 + e[..., 2] ** 2)),
+           rel(p - e[..., 3], e[..., 3])]
+    for ab in ((0, 0), (0, 1), (1, 1), (1, 0)):
+        out.append(rel(ex_s[ab] - sig[..., ab[0], ab[1]], ex_s[ab]))
+    out += [delta * np.sum(rho), delta * np.sum(rho * (d[..., 0] + d[..., 1])),
+            delta * np.sum(rho * (d[..., 0] ** 2 + d[..., 1] ** 2)),
+            float(np.sum(~np.all(np.isfinite(f), axis=2)))]
+    return np.array(out)
+
+
+def velocity_stats(q, f, u_old=None):
+    """Stopping-criteria sums (stopping_criteria.jl:17-99): mean ux, mean uy,
+    sum |u - u_old|^2, sum |u_old|^2, non-finite nodes."""
+    rho, ux, uy = moments_field(q, f)
+    out = [ux.mean(), uy.mean(), 0.0, 0.0, float(np.sum(~np.all(np.isfinite(f), axis=2)))]
+    if u_old is not None:
+        out[2] = float(np.sum((ux - u_old[..., 0]) ** 2 + (uy - u_old[..., 1]) ** 2))
+        out[3] = float(np.sum(u_old ** 2))
+    return np.array(out)

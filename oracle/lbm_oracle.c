@@ -609,3 +609,137 @@ void lbo_moments(const lbo_quad *q, int nx, int ny, const double *fin, double ta
             if (syy_o) syy_o[n] = sg[3] - tr2;
         }
 }
+
+/* ------------------------------------------------------------------------ */
+/* Processing: one row of next!(::TrackHydrodynamicErrors)                  */
+/* (processing_methods/track_hydrodynamic_errors.jl:52-222) and the sums of  */
+/* the stopping criteria (processing_methods/stopping_criteria/              */
+/* stopping_criteria.jl:17-115).  The sums run sequentially in the           */
+/* reference's loop order `for x_idx in 1:nx, y_idx in 1:ny` (x outer).     */
+/* expected: 8 planes rho, u_x, u_y, p, sigma_xx, sigma_xy, sigma_yx,        */
+/* sigma_yy; expected field k = affine[2k] + affine[2k+1] * plane k.         */
+/* ------------------------------------------------------------------------ */
+void lbo_track_errors(const lbo_quad *q, int nx, int ny, const double *fin,
+                      const double *expected, const double *affine, double tau, double u_max,
+                      double delta, double *out) {
+    const int Q = q->Q;
+    const double cs = 1.0 / q->s;
+    const size_t plane = (size_t)nx * (size_t)ny;
+    double H2[LBO_QMAX][4];
+    for (int i = 0; i < Q; ++i) hermite_tensor(2, q->cx[i], q->cy[i], cs, H2[i]);
+    double err_rho = 0.0, err_u = 0.0, err_p = 0.0;
+    double err_sxx = 0.0, err_sxy = 0.0, err_syy = 0.0, err_syx = 0.0;
+    double tot_u2 = 0.0, tot_sxx = 0.0, tot_syy = 0.0, tot_sxy = 0.0, tot_syx = 0.0, tot_p2 = 0.0;
+    double mass = 0.0, momentum = 0.0, energy = 0.0, nonfinite = 0.0;
+    for (int x = 0; x < nx; ++x)
+        for (int y = 0; y < ny; ++y) {
+            const size_t n = (size_t)x + (size_t)nx * (size_t)y;
+            double e[8];
+            for (int k = 0; k < 8; ++k) e[k] = affine[2 * k] + affine[2 * k + 1] * expected[plane * k + n];
+            /* lines 127-132 */
+            tot_u2 += e[1] * e[1] + e[2] * e[2];
+            tot_sxx += e[4] * e[4];
+            tot_syy += e[7] * e[7];
+            tot_sxy += e[5] * e[5];
+            tot_syx += e[6] * e[6];
+            tot_p2 += e[3] * e[3];
+            double f[LBO_QMAX], u[2];
+            int finite = 1;
+            for (int i = 0; i < Q; ++i) {
+                f[i] = fin[IDX(x, y, i)];
+                finite = finite && isfinite(f[i]);
+            }
+            if (!finite) nonfinite += 1.0;
+            const double rho = lbo_density(q, f);
+            lbo_velocity(q, f, rho, u);
+            /* a_bar_2, a_eq_2 (lines 149-150) */
+            double abar[4], aeq[4];
+            for (int k = 0; k < 4; ++k) {
+                double a = f[0] * H2[0][k] + f[1] * H2[1][k];
+                for (int i = 2; i < Q; ++i) a = a + f[i] * H2[i][k];
+                abar[k] = a;
+            }
+            eq_coefficient(2, rho, u, aeq);
+            /* a_2 (line 153), P = a_2 - rho (u u' - I) (line 173), p = tr(P)/D (line 181) */
+            const double c = 1.0 / (2.0 * tau);
+            const double den = 1.0 + 1.0 / (2.0 * tau);
+            double P[4];
+            for (int k = 0; k < 4; ++k) {
+                const int a = k & 1, b = k >> 1;
+                const double a2 = (abar[k] + c * aeq[k]) / den;
+                P[k] = a2 - rho * (a == b ? u[a] * u[b] - 1.0 : u[a] * u[b]);
+            }
+            const double p = ((0.0 + P[0]) + P[3]) / 2.0;
+            /* sigma_lb = deviatoric_tensor(q, tau, f, rho, u) (line 177, moments.jl:81-96) */
+            double sg[4];
+            for (int k = 0; k < 4; ++k) sg[k] = (abar[k] - aeq[k]) / den;
+            const double tr2 = ((0.0 + sg[0]) + sg[3]) / 2.0;
+            sg[0] = sg[0] - tr2;
+            sg[3] = sg[3] - tr2;
+            /* dimensionless_* (problems.jl:126-134), lines 184-187 */
+            const double d1 = u[0] / u_max, d2 = u[1] / u_max;
+            const double sfac = 1.0 / (u_max * u_max);
+            for (int k = 0; k < 4; ++k) sg[k] = sg[k] * sfac;
+            /* lines 192-202, cell weight Δ = 1 */
+            err_p += (p - e[3]) * (p - e[3]);
+            err_rho += (rho - e[0]) * (rho - e[0]);
+            err_u += (d1 - e[1]) * (d1 - e[1]) + (d2 - e[2]) * (d2 - e[2]);
+            err_sxx += (e[4] - sg[0]) * (e[4] - sg[0]);
+            err_sxy += (e[5] - sg[2]) * (e[5] - sg[2]);
+            err_syx += (e[6] - sg[1]) * (e[6] - sg[1]);
+            err_syy += (e[7] - sg[3]) * (e[7] - sg[3]);
+            mass += rho;
+            momentum += rho * (d1 + d2);
+            energy += rho * (d1 * d1 + d2 * d2);
+        }
+    /* lines 205-221 */
+    out[0] = sqrt(err_rho);
+    out[1] = sqrt(err_u / tot_u2);
+    out[2] = sqrt(err_p / tot_p2);
+    out[3] = sqrt(err_sxx / tot_sxx);
+    out[4] = sqrt(err_sxy / tot_sxy);
+    out[5] = sqrt(err_syy / tot_syy);
+    out[6] = sqrt(err_syx / tot_syx);
+    out[7] = delta * mass;
+    out[8] = delta * momentum;
+    out[9] = delta * energy;
+    out[10] = nonfinite;
+}
+
+/* MeanVelocityStoppingCriteria (stopping_criteria.jl:17-39): out[0..1] mean
+ * u_x, u_y; VelocityConvergenceStoppingCriteria (stopping_criteria.jl:71-99)
+ * with u_old (2 planes, may be NULL; updated to u): out[2] sum |u - u_old|^2,
+ * out[3] sum |u_old|^2; out[4] nodes with a non-finite population. */
+void lbo_velocity_stats(const lbo_quad *q, int nx, int ny, const double *fin, double *u_old,
+                        double *out) {
+    const int Q = q->Q;
+    const size_t plane = (size_t)nx * (size_t)ny;
+    double sx = 0.0, sy = 0.0, du2 = 0.0, old2 = 0.0, nonfinite = 0.0;
+    for (int x = 0; x < nx; ++x)
+        for (int y = 0; y < ny; ++y) {
+            const size_t n = (size_t)x + (size_t)nx * (size_t)y;
+            double f[LBO_QMAX], u[2];
+            int finite = 1;
+            for (int i = 0; i < Q; ++i) {
+                f[i] = fin[IDX(x, y, i)];
+                finite = finite && isfinite(f[i]);
+            }
+            if (!finite) nonfinite += 1.0;
+            const double rho = lbo_density(q, f);
+            lbo_velocity(q, f, rho, u);
+            sx += u[0];
+            sy += u[1];
+            if (u_old) {
+                const double o1 = u_old[n], o2 = u_old[plane + n];
+                du2 += (u[0] - o1) * (u[0] - o1) + (u[1] - o2) * (u[1] - o2);
+                old2 += o1 * o1 + o2 * o2;
+                u_old[n] = u[0];
+                u_old[plane + n] = u[1];
+            }
+        }
+    out[0] = sx / (double)plane;
+    out[1] = sy / (double)plane;
+    out[2] = du2;
+    out[3] = old2;
+    out[4] = nonfinite;
+}

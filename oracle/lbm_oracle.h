@@ -73,6 +73,18 @@ int lbo_simulate(const lbo_quad *q, int nx, int ny, double *f_stream, double *f_
 void lbo_moments(const lbo_quad *q, int nx, int ny, const double *f, double tau,
                  double *rho, double *ux, double *uy, double *sxx, double *sxy, double *syy);
 
+/* one row of TrackHydrodynamicErrors: out[11] = error_rho, error_u, error_p,
+ * error_sigma_xx, _xy, _yy, _yx, mass, momentum, energy, non-finite nodes;
+ * expected: 8 planes (rho, ux, uy, p, sxx, sxy, syx, syy), field k =
+ * affine[2k] + affine[2k+1] * plane k */
+void lbo_track_errors(const lbo_quad *q, int nx, int ny, const double *f,
+                      const double *expected, const double *affine, double tau, double u_max,
+                      double delta, double *out);
+/* stopping-criteria sums: out[5] = mean ux, mean uy, sum |u-u_old|^2,
+ * sum |u_old|^2, non-finite nodes; u_old (2 planes) may be NULL, else set to u */
+void lbo_velocity_stats(const lbo_quad *q, int nx, int ny, const double *f, double *u_old,
+                        double *out);
+
 /* OpenMP thread count used by the node loops (1 = scalar) */
 void lbo_set_threads(int n);
 int lbo_get_threads(void);

@@ -1,0 +1,167 @@
+"""GPU parity of the on-device processing: lbm_track_errors and
+lbm_velocity_stats (through the C ABI) against the oracle's sequential sums
+on the same populations.
+
+Every per-node value is formed in the oracle's order, so only the order of
+the FP64 sums differs (a fixed tree on the GPU, a left fold on the CPU): the
+bar is 1e-12 relative for the norms and sums, with an absolute floor scaled
+by the sum of magnitudes where a sum cancels (momentum, mean velocity).
+For the reduced storages the oracle reads the populations as downloaded,
+which is the same FP64 reconstruction the kernel makes.
+"""
+import numpy as np
+import pytest
+
+from lbm_cases import QIDS, random_populations
+
+pytestmark = pytest.mark.gpu
+
+STORAGES = ["f64", "f32", "f16dev", "bf16dev", "f16neq", "bf16neq"]
+
+
+def _L():
+    import lbmhip
+    return lbmhip
+
+
+def _q(qid):
+    return _L().Quadratures[f"D2Q{qid}"]()
+
+
+def _inputs(qid, nx, ny, seed):
+    f = random_populations(qid, nx, ny, seed=seed)
+    rng = np.random.default_rng(seed + 100)
+    E = np.asfortranarray(rng.standard_normal((nx, ny, 8)))
+    aff = rng.standard_normal(16)
+    return f, E, aff
+
+
+def _check_track(got, ref, what):
+    np.testing.assert_allclose(got[:7], ref[:7], rtol=1e-12, err_msg=what)
+    np.testing.assert_allclose(got[7], ref[7], rtol=1e-12, err_msg=what)
+    scale = max(abs(ref[9]), abs(ref[7])) * 1e-12
+    np.testing.assert_allclose(got[8], ref[8], rtol=1e-12, atol=scale, err_msg=what)
+    np.testing.assert_allclose(got[9], ref[9], rtol=1e-12, err_msg=what)
+    assert got[10] == ref[10], what
+
+
+@pytest.mark.parametrize("qid", QIDS)
+@pytest.mark.parametrize("storage", STORAGES)
+def test_track_errors_matches_oracle(oracle_lib, qid, storage):
+    L = _L()
+    nx, ny = 37, 23
+    f, E, aff = _inputs(qid, nx, ny, seed=qid)
+    q = _q(qid)
+    lat = L.HipLattice.from_numpy(q, f, storage=storage)
+    fd = lat.to_numpy()                     # the stored state, in FP64
+    ex = L.ExpectedFields(nx, ny, E)
+    got = L.track_errors(lat, ex, 0.6, 0.03, 0.25, aff)
+    ref = oracle_lib.track_errors(qid, fd, E, aff, 0.6, 0.03, 0.25)
+    _check_track(got, ref, f"D2Q{qid} {storage}")
+
+
+@pytest.mark.parametrize("shape", [(1, 1), (1, 300), (300, 1), (257, 129), (1024, 1100)])
+def test_track_errors_sizes(oracle_lib, shape):
+    """ragged sizes, single rows and columns, and a lattice where every thread
+    of the fixed 2048-block grid visits several nodes"""
+    L = _L()
+    nx, ny = shape
+    f, E, aff = _inputs(9, nx, ny, seed=nx + ny)
+    q = _q(9)
+    lat = L.HipLattice.from_numpy(q, f)
+    ex = L.ExpectedFields(nx, ny, E)
+    got = L.track_errors(lat, ex, 0.8, 0.01, 1.0, aff)
+    ref = oracle_lib.track_errors(9, f, E, aff, 0.8, 0.01, 1.0)
+    _check_track(got, ref, f"{shape}")
+    # the same call twice gives the same bits
+    assert np.array_equal(got, L.track_errors(lat, ex, 0.8, 0.01, 1.0, aff))
+
+
+@pytest.mark.parametrize("qid", [9, 37])
+@pytest.mark.parametrize("storage", ["f64", "f16neq"])
+def test_velocity_stats_and_tracker(oracle_lib, qid, storage):
+    L = _L()
+    nx, ny = 45, 31
+    q = _q(qid)
+    f1 = random_populations(qid, nx, ny, seed=1)
+    f2 = random_populations(qid, nx, ny, seed=2)
+    a = L.HipLattice.from_numpy(q, f1, storage=storage)
+    b = L.HipLattice.from_numpy(q, f2, storage=storage)
+    d1, d2 = a.to_numpy(), b.to_numpy()
+    tr = L.VelocityTracker(nx, ny)
+    u_old = np.zeros((nx, ny, 2), order="F")
+    for lat, fd in ((a, d1), (b, d2), (b, d2)):
+        got = L.velocity_stats(lat, tr)
+        ref = oracle_lib.velocity_stats(qid, fd, u_old)
+        np.testing.assert_allclose(got[2:4], ref[2:4], rtol=1e-12, atol=1e-30)
+        np.testing.assert_allclose(got[:2], ref[:2], rtol=1e-11, atol=1e-15)
+        assert got[4] == ref[4] == 0
+    # without a tracker only the means
+    got = L.velocity_stats(a)
+    ref = oracle_lib.velocity_stats(qid, d1)
+    np.testing.assert_allclose(got[:2], ref[:2], rtol=1e-11, atol=1e-15)
+    assert got[2] == 0 and got[3] == 0
+
+
+def test_nonfinite_population_propagates():
+    L = _L()
+    nx, ny = 20, 9
+    q = _q(9)
+    f = random_populations(9, nx, ny, seed=3)
+    f[4, 5, 2] = np.nan
+    lat = L.HipLattice.from_numpy(q, f)
+    ex = L.ExpectedFields(nx, ny, np.ones((nx, ny, 8)))
+    out = L.track_errors(lat, ex, 0.7, 0.02)
+    assert out[10] == 1 and np.all(np.isnan(out[:10]))
+    s = L.velocity_stats(lat)
+    assert s[4] == 1 and np.isnan(s[0])
+
+
+def test_processing_errors():
+    L = _L()
+    q = _q(9)
+    lat = L.HipLattice(q, 8, 6)
+    with pytest.raises(L.LbmError):
+        L.track_errors(lat, L.ExpectedFields(8, 7), 0.7, 0.02)
+    with pytest.raises(L.LbmError):
+        L.velocity_stats(lat, L.VelocityTracker(6, 8))
+    with pytest.raises(L.LbmError):
+        L.ExpectedFields(0, 5)
+    with pytest.raises(ValueError):
+        L.ExpectedFields(8, 6, np.zeros((8, 6, 7)))
+
+
+def test_track_hydrodynamic_errors_over_a_decaying_vortex(oracle_lib):
+    """simulate(model, 0:250) with TrackHydrodynamicErrors on the GPU: the rows
+    at steps 100, 200 and 250 equal the oracle's rows on the oracle's own FP64
+    run (bit-exact populations, so only the summation order differs), and the
+    populations equal a run without processing."""
+    L = _L()
+    pb = L.TaylorGreenVortex(scale=2, static=False)      # 32 x 32, ν = 1/6
+    q = L.D2Q9()
+    f0 = pb.initial_populations(q)
+    tau = q.speed_of_sound_squared * pb.ν + 0.5
+    pm = L.TrackHydrodynamicErrors(pb, should_process=False, n_steps=250)
+    m = L.LatticeBoltzmannModel.from_populations(q, f0, L.SRT(tau), processing_method=pm,
+                                                 delta_t=pb.delta_t())
+    L.simulate(m, range(0, 251))
+    assert [r.timestep for r in pm.df] == [100, 200, 250]
+    m2 = L.LatticeBoltzmannModel.from_populations(q, f0, L.SRT(tau))
+    L.simulate(m2, range(0, 251))
+    assert np.array_equal(m.f_stream.to_numpy(), m2.f_stream.to_numpy())
+    # the oracle's run and rows
+    fs = np.asfortranarray(f0.copy())
+    fc = np.asfortranarray(f0.copy())
+    done = 0
+    planes = np.asfortranarray(pb.expected_planes(q))
+    x, y = pb.range()
+    delta = ((x[-1] - x[0]) / (len(x) - 1)) * ((y[-1] - y[0]) / (len(y) - 1))
+    for row in pm.df:
+        oracle_lib.simulate(9, fs, fc, 0, [tau], row.timestep - done)
+        done = row.timestep
+        aff = pb.expected_affine(q, row.timestep * pb.delta_t())
+        ref = oracle_lib.track_errors(9, fs, planes, aff, q.speed_of_sound_squared * pb.ν,
+                                      pb.u_max, delta)
+        _check_track(np.array(list(row[1:]) + [0.0]), ref, f"t={row.timestep}")
+    # the lattice solution tracks the analytic vortex: small velocity error
+    assert all(r.error_u < 2e-2 for r in pm.df)
