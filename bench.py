@@ -133,8 +133,10 @@ def processing_leg(L, q, model, n, u0, tau_s, t_steps, calls=20):
         "host_path": "download the populations + the oracle's node loop on 1 core",
         "download_ms": round(down_ms, 1),
         "speedup_vs_host_path": round((down_ms + host_ms) / dev_ms, 1),
-        "max_rel_diff_vs_oracle": float(np.max(np.abs(row[:10] - ref[:10])
-                                               / np.maximum(np.abs(ref[:10]), 1e-300))),
+        # (error_sigma_xy is 0/0 for this vortex, whose analytic sigma_xy vanishes)
+        "max_rel_diff_vs_oracle": float(np.max([abs(a - b) / max(abs(b), 1e-300)
+                                                for a, b in zip(row[:10], ref[:10])
+                                                if np.isfinite(b)])),
     })
     return out
 

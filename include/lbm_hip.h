@@ -45,8 +45,10 @@ enum { LBM_D2Q4 = 4, LBM_D2Q5 = 5, LBM_D2Q9 = 9, LBM_D2Q13 = 13,
 enum { LBM_STORE_F64 = 0, LBM_STORE_F32 = 1, LBM_STORE_F16DEV = 2, LBM_STORE_BF16DEV = 3,
        LBM_STORE_F16NEQ = 4, LBM_STORE_BF16NEQ = 5 };
 
-/* collision models: src/collision_models/{srt,trt,mrt}.jl */
-enum { LBM_SRT = 0, LBM_TRT = 1, LBM_MRT = 2 };
+/* collision models: src/collision_models/{srt,trt,mrt}.jl; LBM_MEI is the
+ * collide-only IterativeInitializationCollisionModel
+ * (collision_models/iterative_initialization.jl) */
+enum { LBM_SRT = 0, LBM_TRT = 1, LBM_MRT = 2, LBM_MEI = 3 };
 
 /* body force, evaluated on the host (the reference's per-node closure
  * srt.jl:52, trt.jl:77, mrt.jl:92 cannot cross the ABI) */
@@ -94,6 +96,8 @@ int lbm_field_upload(lbm_field f, const double *host);
 int lbm_field_download(lbm_field f, double *host);
 /* dst .= src (same shape and storage), a device copy kernel */
 int lbm_field_copy(lbm_field dst, lbm_field src);
+/* f[x, y, :] for 1-based (x, y): the Q populations of one node in FP64 */
+int lbm_field_node(lbm_field f, int x, int y, double *out);
 
 /* ---- model (collision model, force, boundary conditions) --------------- */
 int lbm_model_create(lbm_model *out, int quadrature);
@@ -105,6 +109,12 @@ int lbm_model_set_srt(lbm_model m, double tau);
 int lbm_model_set_trt(lbm_model m, double tau_symmetric, double tau_asymmetric);
 /* MRT(q, taus) -- mrt.jl:19-34; taus[n-1] relaxes Hermite order n */
 int lbm_model_set_mrt(lbm_model m, const double *taus, int ntaus);
+/* IterativeInitializationCollisionModel(q, tau, problem)
+ * (iterative_initialization.jl:12-58): u = the lattice velocity (nx, ny, 2)
+ * column-major, from which the nonlinear equilibrium term is formed; collide!
+ * then relaxes towards w_i rho + that term.  FP64 storage; lbm_simulate runs
+ * it as collide! + stream! + apply! per step. */
+int lbm_model_set_mei(lbm_model m, double tau, const double *u, int nx, int ny);
 /* LBM_FORCE_NONE; LBM_FORCE_CONST: values = {Fx, Fy}; LBM_FORCE_FIELD:
  * values = (nx, ny, 2) column-major lattice force, copied to the device. */
 int lbm_model_set_force(lbm_model m, int kind, const double *values, int nx, int ny);
@@ -114,7 +124,7 @@ int lbm_model_set_bcs(lbm_model m, const lbm_bc *bcs, int n);
 /* Launch shape of the fused step kernel: threads per block along x (64, 128,
  * 256), lattice rows per block (1, 2, 4; block_x * rows <= 256), lattice
  * nodes per thread (1, 2, 4, 8; along x, except for the non-equilibrium
- * storages on D2Q4/5/9, where they are rows marched by one thread: 1..16) and
+ * storages, where more than one are rows marched by one thread: 2..16) and
  * non-temporal stores (0/1);
  * 0 (or -1 for nontemporal) = automatic.  No counterpart in the reference (a
  * CPU loop has no launch shape). */
@@ -202,6 +212,18 @@ int lbm_adjoint_loss(lbm_adjoint a, const double *u0, const double *u_target, do
  * grad_tau[0..1] = dL/dtau_s, dL/dtau_a (TRT); *ms (may be NULL) = device time */
 int lbm_adjoint_gradient(lbm_adjoint a, const double *u0, const double *u_target, double *loss,
                          double *grad_u0, double *grad_tau, double *ms);
+
+/* ---- initialization --------------------------------------------------- */
+/* initialize(IterativeInitializationMeiEtAl(tau, eps), q, problem)
+ * (initial_conditions/mei_et_al.jl:13-41) with a model set by
+ * lbm_model_set_mei and the problem's BC list: both fields start at f = w,
+ * then up to max_steps steps of the model's collide! + stream! + apply!, each
+ * followed by the DensityConvergence check of ProcessIterativeInitialization
+ * (process_iterative_initialization.jl:13-22, density_convergence.jl:6-17:
+ * stop when the density of node (nx, ny) changed by less than eps or more than
+ * 100).  f_stream holds the result; *steps (may be NULL) the steps run. */
+int lbm_mei_initialize(lbm_model m, lbm_field f_stream, lbm_field f_collision, double eps,
+                       int64_t max_steps, int64_t *steps);
 
 /* ---- moments ------------------------------------------------------------ */
 /* density / velocity! / deviatoric_tensor per node (moments.jl:1-19,81-96),

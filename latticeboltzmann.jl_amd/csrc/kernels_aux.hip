@@ -324,26 +324,42 @@ k_track(const void *f, const StepParams<double> p, const TrackArgs a,
         const int nw = (int)blockDim.x >> 6;
         double t = red[0][threadIdx.x];
         for (int w = 1; w < nw; ++w) t += red[w][threadIdx.x];
-        partials[(long long)blockIdx.x * NC + threadIdx.x] = t;
+        // column-major: each sum's partials are contiguous for the final pass
+        partials[(long long)threadIdx.x * gridDim.x + blockIdx.x] = t;
     }
 }
 
-// fixed-order sum of the partial rows, one column after another (one block)
+// fixed-order sum of the partials, one block per column
 __global__ void __launch_bounds__(256)
 k_track_final(const double *__restrict__ partials, int npart, double *__restrict__ out) {
     __shared__ double red[256];
-    for (int k = 0; k < kTrackCols; ++k) {
-        double v = 0.0;
-        for (int b = threadIdx.x; b < npart; b += blockDim.x) v += partials[(long long)b * kTrackCols + k];
-        red[threadIdx.x] = v;
-        __syncthreads();
-        for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-            if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) out[k] = red[0];
+    const double *col = partials + (long long)blockIdx.x * npart;
+    double v = 0.0;
+    for (int b = threadIdx.x; b < npart; b += blockDim.x) v += col[b];
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
         __syncthreads();
     }
+    if (threadIdx.x == 0) out[blockIdx.x] = red[0];
+}
+
+// the Q populations of one node in FP64 (f_in[x, y, :])
+template <int SID>
+__global__ void __launch_bounds__(64)
+k_gather_node(const void *f, const StepParams<double> p, long long n, int Q,
+              double *__restrict__ out) {
+    const int i = threadIdx.x;
+    if (i < Q) out[i] = load_d<SID>(f, p.plane, n, i, p);
+}
+
+// f[plane * i + n] = w_i: the rest state of weight-valued populations
+__global__ void __launch_bounds__(256)
+k_fill_weights(double *__restrict__ f, const StepParams<double> p, int Q) {
+    const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= p.plane) return;
+    for (int i = 0; i < Q; ++i) f[p.plane * i + n] = p.w[i];
 }
 
 template <class U>
@@ -516,6 +532,26 @@ int dispatch_upload(int qid, int storage, const double *src, void *dst,
     return hipGetLastError() == hipSuccess ? 0 : -4;
 }
 
+int dispatch_gather_node(int storage, const void *f, const StepParams<double> &p, long long n,
+                         int Q, double *out, hipStream_t s) {
+    if (Q < 1 || Q > 64 || n < 0 || n >= p.plane) return -1;
+    switch (storage) {
+    case 0: k_gather_node<0><<<1, 64, 0, s>>>(f, p, n, Q, out); break;
+    case 1: k_gather_node<1><<<1, 64, 0, s>>>(f, p, n, Q, out); break;
+    case 2: k_gather_node<2><<<1, 64, 0, s>>>(f, p, n, Q, out); break;
+    case 3: k_gather_node<3><<<1, 64, 0, s>>>(f, p, n, Q, out); break;
+    default: return -1;
+    }
+    return hipGetLastError() == hipSuccess ? 0 : -4;
+}
+
+int dispatch_fill_weights(double *f, const StepParams<double> &p, int Q, hipStream_t s) {
+    if (Q < 1 || Q > 37) return -1;
+    const long long blocks = (p.plane + 255) / 256;
+    k_fill_weights<<<(unsigned)blocks, 256, 0, s>>>(f, p, Q);
+    return hipGetLastError() == hipSuccess ? 0 : -4;
+}
+
 // device copy of nbytes (multiple of 8 for double storage, 4 float, 2 half)
 int dispatch_copy(const void *src, void *dst, size_t nbytes, hipStream_t s) {
     if (nbytes % 8 == 0) {
@@ -569,7 +605,7 @@ int dispatch_track(int qid, int storage, const void *f, const StepParams<double>
     default: return -1;
     }
     if (rc) return rc;
-    k_track_final<<<1, 256, 0, s>>>(partials, npart, out);
+    k_track_final<<<kTrackCols, 256, 0, s>>>(partials, npart, out);
     return hipGetLastError() == hipSuccess ? 0 : -4;
 }
 

@@ -214,6 +214,8 @@ struct lbm_model_s {
     lbm::BCDev *dB = nullptr;
     double *dMw = nullptr;
     double *dH = nullptr;
+    double *dMei = nullptr;  // iterative initialization: nonlinear term (nx, ny, Q)
+    int mei_nx = 0, mei_ny = 0;
     double *dPart = nullptr;  // reduction partials
     size_t npart = 0;
     double *dOut = nullptr;
@@ -253,6 +255,13 @@ int check_pair(lbm_model m, lbm_field a, lbm_field b, const char *what, bool use
     }
     if (uses_force && m->coll == LBM_MRT && a->storage != LBM_STORE_F64)
         return fail(-2, "%s: MRT is implemented for FP64 storage only", what);
+    if (uses_force && m->coll == LBM_MEI) {
+        if (a->storage != LBM_STORE_F64)
+            return fail(-2, "%s: the iterative initialization runs on FP64 storage only", what);
+        if (m->mei_nx != a->nx || m->mei_ny != a->ny)
+            return fail(-1, "%s: initialization velocity is %dx%d but the lattice is %dx%d", what,
+                        m->mei_nx, m->mei_ny, a->nx, a->ny);
+    }
     return 0;
 }
 
@@ -280,6 +289,10 @@ lbm::StepParams<double> make_params(const lbm_model_s *m, const lbm_field_s *f, 
         p.oms = 1.0 / m->tau_s;
         p.oma = 1.0 / m->tau_a;
         p.tau_force = m->tau_a;
+    } else if (m->coll == LBM_MEI) {
+        p.omo = 1.0 - 1.0 / m->tau;
+        p.om = 1.0 / m->tau;
+        p.mei = m->dMei;
     } else {
         p.mrt_n = q.order / 2;
         for (int n = 1; n <= 4; ++n) {
@@ -304,42 +317,49 @@ lbm::StepParams<double> make_params(const lbm_model_s *m, const lbm_field_s *f, 
     return p;
 }
 
-// non-equilibrium storages on one-speed stencils (D2Q4/5/9): nodes per thread
-// count lattice rows (k_fused_neq_rows), any nx
-bool rows_mode(int Q, int storage) {
-    return (storage == LBM_STORE_F16NEQ || storage == LBM_STORE_BF16NEQ) && Q <= 9;
+// non-equilibrium storages: more than one node per thread counts lattice rows
+// (k_fused_neq_rows), any nx
+bool rows_mode(int storage) {
+    return storage == LBM_STORE_F16NEQ || storage == LBM_STORE_BF16NEQ;
 }
 
 // largest nodes-per-thread the kernels instantiate (mirrors max_vec in lbm_kernels.h)
 int max_vec(int Q, int storage) {
-    if (rows_mode(Q, storage)) return 16;
+    if (rows_mode(storage)) return 16;
     const int eb = storage_bytes(storage);
     const int cap = eb >= 8 ? 2 : (eb == 4 ? 4 : 8);
     return Q <= 9 ? cap : (Q <= 17 ? (cap < 4 ? cap : 4) : 2);
 }
 
-// automatic launch shape of the fused step.  Measured on MI355X, D2Q9 TRT
-// 4096^2: non-temporal stores are fastest for every storage; one node per
-// thread for 8- and 4-byte populations, four for the
-// 16-bit deviation storages (one dwordx2 per plane per lane), and for the
-// non-equilibrium storages on one-speed stencils 8 rows per thread (the
-// records of the 3 x 3 neighbourhood move along in registers; 1 node per
-// thread on wider stencils).  lbm_autotune refines this per lattice.
+// automatic launch shape of the fused step.  Measured on MI355X (D2Q9 TRT
+// 4096^2, D2Q17 / D2Q37 8192 x 2048): non-temporal stores are fastest for
+// every storage; one node per thread for 8- and 4-byte populations and for
+// 16-bit deviations on the wide stencils, four for 16-bit deviations on
+// D2Q9 (one dwordx2 per plane per lane); for the non-equilibrium storages 8
+// rows per thread (k_fused_neq_rows, records in an LDS window).
+// lbm_autotune refines this per lattice.
 void auto_launch(const lbm_model_s *m, const lbm_field_s *f, int &block, int &rows, int &vec,
                  int &nt) {
     // 8-byte populations: 128 x 2 blocks (2-3% faster than 256 x 1 for D2Q9 and
     // D2Q17 FP64 on MI355X); everything else 256 x 1
     const bool f64 = storage_bytes(f->storage) == 8 && m->coll != LBM_MRT;
-    block = m->launch_block > 0 ? m->launch_block : (f64 ? 128 : 256);
+    // 16-bit populations on the 21- and 37-velocity stencils: 64-thread blocks
+    // (D2Q37 FP16-DEV 8192 x 2048: 508 us against 534 us with 256)
+    const bool wide16 = storage_bytes(f->storage) == 2 && f->Q > 17;
+    block = m->launch_block > 0 ? m->launch_block : (f64 ? 128 : (wide16 ? 64 : 256));
     rows = m->launch_rows > 0 ? m->launch_rows : (f64 ? 2 : 1);
     if (block * rows > 256) rows = 256 / block;
     vec = m->launch_vec;
     nt = m->launch_nt;
-    if (vec <= 0 && rows_mode(f->Q, f->storage) && m->coll != LBM_MRT) vec = 8;
+    if (vec <= 0 && rows_mode(f->storage) && m->coll != LBM_MRT) vec = 8;
     if (vec <= 0) {
         vec = 1;
         const bool neq = f->storage == LBM_STORE_F16NEQ || f->storage == LBM_STORE_BF16NEQ;
-        if (m->coll != LBM_MRT && storage_bytes(f->storage) == 2 && !neq) {
+        // several nodes per lane only for the 9-velocity stencils: on wider
+        // ones one node per lane with per-population stores is faster (D2Q17
+        // FP16-DEV 8192 x 2048: 216 us against 238 us with 4 nodes; D2Q37: 508
+        // against 636 us with 2)
+        if (m->coll != LBM_MRT && storage_bytes(f->storage) == 2 && !neq && f->Q <= 9) {
             const int want = max_vec(f->Q, f->storage) < 4 ? max_vec(f->Q, f->storage) : 4;
             for (int v = want; v > 1; v >>= 1)
                 if (f->nx % v == 0 && f->nx / v >= 64) {
@@ -590,6 +610,7 @@ int lbm_model_destroy(lbm_model m) {
     free_dev(reinterpret_cast<void *&>(m->dB));
     free_dev(reinterpret_cast<void *&>(m->dMw));
     free_dev(reinterpret_cast<void *&>(m->dH));
+    free_dev(reinterpret_cast<void *&>(m->dMei));
     free_dev(reinterpret_cast<void *&>(m->dPart));
     free_dev(reinterpret_cast<void *&>(m->dOut));
     delete m;
@@ -624,6 +645,44 @@ int lbm_model_set_mrt(lbm_model m, const double *taus, int ntaus) {
         if (!(taus[k] > 0.0)) return fail(-1, "MRT relaxation times must be positive");
     m->coll = LBM_MRT;
     m->taus.assign(taus, taus + ntaus);
+    return 0;
+}
+
+int lbm_model_set_mei(lbm_model m, double tau, const double *u, int nx, int ny) {
+    if (!m) return fail(-1, "null model");
+    if (!(tau > 0.0) || !std::isfinite(tau)) return fail(-1, "initialization tau must be positive");
+    if (!u) return fail(-1, "initialization velocity is null");
+    if (nx < 1 || ny < 1) return fail(-1, "initialization velocity size must be positive");
+    if ((long long)nx * (long long)ny > (1LL << 31) - 1)
+        return fail(-1, "lattice of %lld nodes exceeds 2^31-1", (long long)nx * ny);
+    if (int rc = ensure_stream()) return rc;
+    // nonlinear_term (iterative_initialization.jl:12-40) in the reference's order
+    const HostQuad &q = m->q;
+    const size_t plane = (size_t)nx * (size_t)ny;
+    std::vector<double> N(plane * (size_t)q.Q);
+    const double cs = q.s;
+    for (size_t n = 0; n < plane; ++n) {
+        const double u0 = u[n], u1 = u[plane + n];
+        const double usq = u0 * u0 + u1 * u1;
+        for (int i = 0; i < q.Q; ++i) {
+            const double ud = (0.0 + (double)q.cx[i] * u0) + (double)q.cy[i] * u1;
+            const double a1 = cs * ud;
+            const double a2 = (cs * cs) * (ud * ud) - cs * usq;
+            N[plane * (size_t)i + n] = (q.w[i] * 1.0) * (a1 + a2 / 2.0);
+        }
+    }
+    const size_t bytes = N.size() * sizeof(double);
+    if (m->mei_nx != nx || m->mei_ny != ny || !m->dMei) {
+        free_dev(reinterpret_cast<void *&>(m->dMei));
+        m->mei_nx = m->mei_ny = 0;
+        HIP_TRY(hipMalloc(&m->dMei, bytes));
+    }
+    HIP_TRY(hipMemcpyAsync(m->dMei, N.data(), bytes, hipMemcpyHostToDevice, g_stream));
+    HIP_TRY(hipStreamSynchronize(g_stream));
+    m->mei_nx = nx;
+    m->mei_ny = ny;
+    m->coll = LBM_MEI;
+    m->tau = tau;
     return 0;
 }
 
@@ -723,6 +782,16 @@ static int simulate_impl(lbm_model m, lbm_field S, lbm_field C, int64_t nsteps, 
     if (ms) *ms = 0.0;
     if (nlaunch) *nlaunch = 0;
     if (nsteps == 0) return 0;
+    if (m->coll == LBM_MEI) {
+        // collide-only model: collide! then stream! + apply! each step, in the
+        // reference's roles (no fused kernel)
+        for (int64_t k = 0; k < nsteps; ++k) {
+            if (int rc = run(lbm::Op::Collide, m, S, C, false)) return rc;
+            if (int rc = run(lbm::Op::StreamBC, m, C, S, true)) return rc;
+        }
+        if (nlaunch) *nlaunch = 2 * nsteps;
+        return 0;
+    }
     // step 1 starts from the canonical state S: C_0 = collide(S_0)
     if (int rc = run(lbm::Op::Collide, m, S, C, false)) return rc;
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -802,7 +871,7 @@ int lbm_autotune(lbm_model m, lbm_field f_stream, int trial_steps, int *block_x,
     const int mv = (m->coll == LBM_MRT) ? 1 : max_vec(f_stream->Q, f_stream->storage);
     // (threads along x, rows) per block
     const int shapes[6][2] = {{64, 1}, {128, 1}, {256, 1}, {64, 4}, {128, 2}, {64, 2}};
-    const bool rows = rows_mode(f_stream->Q, f_stream->storage);
+    const bool rows = rows_mode(f_stream->storage);
     for (int vec = 1; !rc && vec <= mv; vec <<= 1) {
         if (!rows && f_stream->nx % vec) continue;
         for (int nt = 0; !rc && nt <= (m->coll == LBM_MRT ? 0 : 1); ++nt) {
@@ -944,6 +1013,79 @@ int lbm_reduce(lbm_field f, double *out5) {
     free_dev(reinterpret_cast<void *&>(ftmp));
     if (e != hipSuccess) return fail(-3, "lbm_reduce: %s", hipGetErrorString(e));
     if (rc) return fail(-4, "reduction kernel failed (%d)", rc);
+    return 0;
+}
+
+int lbm_field_node(lbm_field f, int x, int y, double *out) {
+    if (int rc = check_field(f, "lbm_field_node")) return rc;
+    if (!out) return fail(-1, "lbm_field_node: null output");
+    if (x < 1 || x > f->nx || y < 1 || y > f->ny)
+        return fail(-1, "lbm_field_node: node (%d, %d) outside the %dx%d lattice", x, y, f->nx,
+                    f->ny);
+    const lbm::StepParams<double> p = field_params(f);
+    const void *fdata;
+    int fstorage;
+    double *ftmp;
+    if (int rc = as_f64(f, &fdata, &fstorage, &ftmp)) return rc;
+    double *dout = nullptr;
+    hipError_t e = hipMalloc(&dout, 64 * sizeof(double));
+    int rc = 0;
+    const long long n = (long long)(x - 1) + (long long)f->nx * (y - 1);
+    if (e == hipSuccess) rc = lbm::dispatch_gather_node(fstorage, fdata, p, n, f->Q, dout, g_stream);
+    if (e == hipSuccess && !rc)
+        e = hipMemcpyAsync(out, dout, (size_t)f->Q * sizeof(double), hipMemcpyDeviceToHost,
+                           g_stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(g_stream);
+    free_dev(reinterpret_cast<void *&>(dout));
+    free_dev(reinterpret_cast<void *&>(ftmp));
+    if (e != hipSuccess) return fail(-3, "lbm_field_node: %s", hipGetErrorString(e));
+    if (rc) return fail(-4, "lbm_field_node: gather kernel failed (%d)", rc);
+    return 0;
+}
+
+int lbm_mei_initialize(lbm_model m, lbm_field f_stream, lbm_field f_collision, double eps,
+                       int64_t max_steps, int64_t *steps) {
+    if (int rc = check_pair(m, f_stream, f_collision, "lbm_mei_initialize", true, true)) return rc;
+    if (m->coll != LBM_MEI)
+        return fail(-1, "lbm_mei_initialize: the model has no initialization velocity "
+                        "(lbm_model_set_mei)");
+    if (max_steps < 1) return fail(-1, "lbm_mei_initialize: max_steps must be positive");
+    const lbm::StepParams<double> p = field_params(f_stream);
+    // f = w everywhere (mei_et_al.jl:21-25), in both roles
+    int rc = lbm::dispatch_fill_weights(static_cast<double *>(f_stream->d), p, f_stream->Q, g_stream);
+    if (!rc) rc = lbm::dispatch_fill_weights(static_cast<double *>(f_collision->d), p,
+                                             f_stream->Q, g_stream);
+    if (rc) return fail(-4, "lbm_mei_initialize: fill kernel failed (%d)", rc);
+    double *dnode = nullptr;
+    HIP_TRY(hipMalloc(&dnode, 64 * sizeof(double)));
+    const long long corner = (long long)(f_stream->nx - 1) + (long long)f_stream->nx * (f_stream->ny - 1);
+    double rho = 0.0, rho_old = 0.0, node[64];
+    int64_t t = 1;
+    hipError_t e = hipSuccess;
+    for (; t <= max_steps; ++t) {
+        if ((rc = run(lbm::Op::Collide, m, f_stream, f_collision, false))) break;
+        if ((rc = run(lbm::Op::StreamBC, m, f_collision, f_stream, true))) break;
+        // next!(ProcessIterativeInitialization): DensityConvergence at node (nx, ny)
+        if ((rc = lbm::dispatch_gather_node(LBM_STORE_F64, f_stream->d, p, corner, f_stream->Q,
+                                            dnode, g_stream))) {
+            rc = fail(-4, "lbm_mei_initialize: gather kernel failed (%d)", rc);
+            break;
+        }
+        e = hipMemcpyAsync(node, dnode, (size_t)f_stream->Q * sizeof(double),
+                           hipMemcpyDeviceToHost, g_stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(g_stream);
+        if (e != hipSuccess) break;
+        double r = node[0] + node[1];
+        for (int i = 2; i < f_stream->Q; ++i) r = r + node[i];
+        rho_old = rho;
+        rho = r;
+        const double d = std::fabs(rho - rho_old);
+        if (d < eps || d > 100.0) break;
+    }
+    free_dev(reinterpret_cast<void *&>(dnode));
+    if (e != hipSuccess) return fail(-3, "lbm_mei_initialize: %s", hipGetErrorString(e));
+    if (rc) return rc;
+    if (steps) *steps = t > max_steps ? max_steps : t;
     return 0;
 }
 

@@ -44,6 +44,8 @@ struct StepParams {
     T mrt_sn[5];             // s^n
     const double *mrt_H;     // Hermite tables (see lbm_capi.cpp)
     T cs_inv;                // 1/s
+    // iterative initialization (FP64 only): nonlinear equilibrium term (nx, ny, Q)
+    const double *mei;
 };
 
 // FP32 copy of the parameters for the FP32-arithmetic storages
@@ -78,6 +80,7 @@ inline StepParams<float> to_float_params(const StepParams<double> &d) {
     f.mrt_n = 0;
     f.mrt_H = nullptr;
     f.cs_inv = static_cast<float>(d.cs_inv);
+    f.mei = nullptr;
     return f;
 }
 
@@ -103,7 +106,7 @@ enum class Op : int { Collide = 0, StreamBC = 1, Fused = 2, ApplyBC = 3, Copy = 
 struct Launch {
     Op op;
     int qid;
-    int coll;     // 0 SRT, 1 TRT, 2 MRT
+    int coll;     // 0 SRT, 1 TRT, 2 MRT, 3 iterative initialization (collide only)
     int storage;  // LBM_STORE_*
     const void *src;
     void *dst;
@@ -184,5 +187,10 @@ int dispatch_track(int qid, int storage, const void *f, const StepParams<double>
 int dispatch_upload(int qid, int storage, const double *src, void *dst,
                     const StepParams<double> &p, int to_device, hipStream_t s);
 int dispatch_copy(const void *src, void *dst, size_t nbytes, hipStream_t s);
+// the Q populations of node n in FP64 (plain and DEV storages)
+int dispatch_gather_node(int storage, const void *f, const StepParams<double> &p, long long n,
+                         int Q, double *out, hipStream_t s);
+// f[plane * i + n] = w_i for every node (plain FP64 storage)
+int dispatch_fill_weights(double *f, const StepParams<double> &p, int Q, hipStream_t s);
 
 }  // namespace lbm

@@ -324,6 +324,66 @@ __device__ __forceinline__ float geq_rec(const Mom4 &m, const StepParams<float> 
     }
 }
 
+// A node's own shifted equilibrium, shared by all of its populations.  For
+// FEQ >= 3 the expansion of maxwell_boltzmann_equilibrium.jl in t = c_i.u,
+//   s t + (s^2 t^2 - s u^2)/2 + s t (s^2 t^2 - 3 s u^2)/6
+//       + (s^4 t^4 - 6 s^3 u^2 t^2 + 3 s^2 u^4)/24,
+// is a polynomial in t with per-node coefficients; with rho folded in,
+// feq_i - w_i rho0 = w_i (k0 + t (k1 + t (k2 + t (k3 + t k4)))): four FMAs per
+// population by Horner's rule.  FEQ <= 2 keeps the record form.
+struct NodeEq {
+    Mom4 rec;
+    float k0, k1, k2, k3, k4, ux, uy;
+};
+
+template <int QID>
+__device__ __forceinline__ NodeEq node_eq(float drho, float ux, float uy,
+                                          const StepParams<float> &p) {
+    constexpr int FEQ = QT<QID>::FEQ;
+    NodeEq e{};
+    if constexpr (FEQ <= 2) {
+        e.rec = make_rec<QID>(drho, ux, uy, p);
+    } else {
+        const float rho = p.rho0 + drho;
+        const float usq = ux * ux + uy * uy;
+        const float s3 = p.s * p.s2;
+        float c0 = -0.5f * p.s * usq;
+        const float c1 = p.s - 0.5f * p.s2 * usq;
+        float c2 = 0.5f * p.s2;
+        const float c3 = (1.0f / 6.0f) * s3;
+        float c4 = 0.0f;
+        if constexpr (FEQ >= 4) {
+            c0 = c0 + 0.125f * p.s2 * (usq * usq);
+            c2 = c2 - 0.25f * s3 * usq;
+            c4 = (1.0f / 24.0f) * p.s4;
+        }
+        e.k0 = drho + rho * c0;
+        e.k1 = rho * c1;
+        e.k2 = rho * c2;
+        e.k3 = rho * c3;
+        e.k4 = rho * c4;
+        e.ux = ux;
+        e.uy = uy;
+    }
+    return e;
+}
+
+template <int QID, int I>
+__device__ __forceinline__ float geq_node(const NodeEq &e, const StepParams<float> &p) {
+    constexpr int FEQ = QT<QID>::FEQ;
+    if constexpr (FEQ <= 2) {
+        return geq_rec<QID, I>(e.rec, p);
+    } else {
+        const float t = udot<QID, I>(e.ux, e.uy);
+        float h = e.k3;
+        if constexpr (FEQ >= 4) h = h + t * e.k4;
+        h = e.k2 + t * h;
+        h = e.k1 + t * h;
+        h = e.k0 + t * h;
+        return p.w[I] * h;
+    }
+}
+
 // shifted moments: drho = sum g, rho u = sum c g (sum_i c_i w_i = 0)
 template <int QID, class T, int... Is>
 __device__ __forceinline__ void moments_dev_seq(const T (&g)[QT<QID>::Q], const StepParams<T> &p,
@@ -384,8 +444,7 @@ __device__ __forceinline__ void node_force(const StepParams<T> &p, int x, int y,
 // ---------------------------------------------------------------------------
 template <int QID, int COLL, bool SHIFT, class T, int... Is>
 __device__ __forceinline__ void collide_seq(T (&f)[QT<QID>::Q], const StepParams<T> &p, int x,
-                                            int y, std::integer_sequence<int, Is...>,
-                                            Mom4 *rec_out = nullptr, T *geq_out = nullptr) {
+                                            int y, std::integer_sequence<int, Is...>) {
     constexpr int Q = QT<QID>::Q;
     T rho, ux, uy, drho = T(0);
     if constexpr (SHIFT) {
@@ -402,13 +461,9 @@ __device__ __forceinline__ void collide_seq(T (&f)[QT<QID>::Q], const StepParams
     }
     T feq[Q];
     if constexpr (SHIFT) {
-        // FP32 storages: the shifted equilibrium from its record (see Mom4)
-        const Mom4 rec = make_rec<QID>(drho, ux, uy, p);
-        ((feq[Is] = geq_rec<QID, Is>(rec, p)), ...);
-        if (rec_out) {
-            *rec_out = rec;
-            ((geq_out[Is] = feq[Is]), ...);
-        }
+        // FP32 storages: the shifted equilibrium of the node (see NodeEq)
+        const NodeEq e = node_eq<QID>(drho, ux, uy, p);
+        ((feq[Is] = geq_node<QID, Is>(e, p)), ...);
     } else {
         const T usq = ux * ux + uy * uy;
         ((feq[Is] = feq_i<QID, Is, T>(rho, ux, uy, usq, p)), ...);
@@ -464,10 +519,11 @@ __device__ __forceinline__ void collide_neq_seq(const float (&f)[QT<QID>::Q],
     const Mom4 mt = make_rec<QID>(drho, ux, uy, p);
     rec = ST::pack(mt);
     const Mom4 mq = ST::unpack(rec);
+    const NodeEq e = node_eq<QID>(drho, ux, uy, p);
     float n[Q], d[Q];
     auto split = [&](auto Ic) {
         constexpr int i = decltype(Ic)::value;
-        const float ge = geq_rec<QID, i>(mt, p);
+        const float ge = geq_node<QID, i>(e, p);
         n[i] = f[i] - ge;
         d[i] = (ge - geq_rec<QID, i>(mq, p)) * p.dscale;
     };
@@ -656,13 +712,32 @@ __device__ __forceinline__ void store_local(typename ST::store_t *__restrict__ F
     }
 }
 
+// collide!(::IterativeInitializationCollisionModel) (iterative_initialization.jl:41-58):
+// f_i = (1 - 1/tau) f_i + (1/tau) (w_i rho + N_i), N the precomputed nonlinear
+// part of the equilibrium of the prescribed velocity; FP64
+template <int QID>
+__device__ __forceinline__ void collide_mei_node(double (&f)[QT<QID>::Q],
+                                                 const StepParams<double> &p, int x, int y) {
+    constexpr int Q = QT<QID>::Q;
+    double rho = f[0] + f[1];
+#pragma unroll
+    for (int i = 2; i < Q; ++i) rho = rho + f[i];
+    const long long n = (long long)x + (long long)p.nx * y;
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+        const double feq = p.w[i] * rho + p.mei[p.plane * i + n];
+        f[i] = p.omo * f[i] + p.om * feq;
+    }
+}
+
 // FP64 runs on the populations themselves (the reference's arithmetic); the
 // FP32 storages on shifted populations
 template <int QID, int COLL, class T>
 __device__ __forceinline__ void collide_any(T (&f)[QT<QID>::Q], const StepParams<T> &p, int x,
                                             int y) {
     constexpr bool SHIFT = !std::is_same<T, double>::value;
-    if constexpr (COLL == 2) collide_mrt_node<QID, T>(f, p, x, y);
+    if constexpr (COLL == 3) collide_mei_node<QID>(f, p, x, y);
+    else if constexpr (COLL == 2) collide_mrt_node<QID, T>(f, p, x, y);
     else collide_node<QID, COLL, SHIFT, T>(f, p, x, y);
 }
 
@@ -881,11 +956,11 @@ __device__ __forceinline__ void collide_store_seq(const typename ST::T (&f)[QT<Q
         uy = uy + p.tau_force * Fy;
     }
     const T usq = ux * ux + uy * uy;
-    Mom4 rec{};
-    if constexpr (SHIFT) rec = make_rec<QID>(drho, ux, uy, p);
+    NodeEq e{};
+    if constexpr (SHIFT) e = node_eq<QID>(drho, ux, uy, p);
     auto feq = [&](auto Ic) -> T {
         constexpr int i = decltype(Ic)::value;
-        if constexpr (SHIFT) return geq_rec<QID, i>(rec, p);
+        if constexpr (SHIFT) return geq_node<QID, i>(e, p);
         else return feq_i<QID, i, T>(rho, ux, uy, usq, p);
     };
     auto put = [&](auto Ic, T v) {
@@ -963,23 +1038,33 @@ k_fused_v(const typename ST::store_t *__restrict__ C, typename ST::store_t *__re
     }
 }
 
-// Edge nodes of a one-speed lattice, enumerated: rows 0 and ny-1, then
-// columns 0 and nx-1 of the rows in between.
-__host__ __device__ inline long long edge_count(int nx, int ny) {
-    return (long long)(ny > 1 ? 2 : 1) * nx + (ny > 2 ? (long long)(nx > 1 ? 2 : 1) * (ny - 2) : 0);
+// The nodes within M of the lattice edge (x < M, x >= nx - M, y < M or
+// y >= ny - M), numbered: first the M bottom and M top rows, then the side
+// columns of the rows between them.  Every other node has all its pull
+// sources inside the lattice.
+__host__ __device__ inline long long edge_count(int nx, int ny, int M) {
+    if (ny <= 2 * M || nx <= 2 * M) return (long long)nx * ny;
+    return 2LL * M * nx + (long long)(ny - 2 * M) * (2 * M);
 }
-__device__ inline bool edge_node(long long k, int nx, int ny, int &x, int &y) {
-    const int nrow = ny > 1 ? 2 : 1, ncol = nx > 1 ? 2 : 1;
-    const long long top = (long long)nrow * nx;
-    if (k < top) {
-        y = k < nx ? 0 : ny - 1;
-        x = (int)(k < nx ? k : k - nx);
+__device__ inline bool edge_node(long long k, int nx, int ny, int M, int &x, int &y) {
+    if (ny <= 2 * M || nx <= 2 * M) {
+        if (k >= (long long)nx * ny) return false;
+        y = (int)(k / nx);
+        x = (int)(k - (long long)y * nx);
         return true;
     }
-    const long long j = k - top;
-    if (ny <= 2 || j >= (long long)ncol * (ny - 2)) return false;
-    y = 1 + (int)(j % (ny - 2));
-    x = j < ny - 2 ? 0 : nx - 1;
+    const long long band = 2LL * M * nx;
+    if (k < band) {
+        const int r = (int)(k / nx);
+        x = (int)(k - (long long)r * nx);
+        y = r < M ? r : ny - 2 * M + r;
+        return true;
+    }
+    const long long j = k - band;
+    if (j >= (long long)(ny - 2 * M) * (2 * M)) return false;
+    const int r = (int)(j / (2 * M)), c = (int)(j - (long long)r * (2 * M));
+    y = M + r;
+    x = c < M ? c : nx - 2 * M + c;
     return true;
 }
 
@@ -1003,35 +1088,38 @@ __device__ __forceinline__ void neq_edge_update(const typename ST::store_t *__re
     adj_each_node<QID>(st);
 }
 
-// Non-equilibrium storages with a one-speed stencil (MAXC = 1): every thread
-// advances R consecutive rows of one lattice column.  The equilibrium
-// records of the three rows around the current one stay in LDS (packed, 8
-// bytes each): four rotating row slots of blockDim.x + 2 records (the block's
-// columns and one halo column on each side).  Every thread loads the record of its own
-// column for the row ahead and one barrier per row publishes it, so a node
-// loads one record instead of one per pulled population (9).  The blocks of
-// the last grid column update the lattice edge instead (general pull: wrap +
-// boundary conditions), a share per row of blocks, so the few edge nodes run
-// alongside the interior rather than as a separate latency-bound launch.  Measured on MI355X (D2Q9 4096^2, FP16-NEQ): 8 rows,
-// 221 us per step, against 260 us with one node per thread (unpacking the
-// records to FP32 in LDS doubles the window and costs occupancy: 243 us).
+// Non-equilibrium storages: every thread advances R consecutive rows of one
+// lattice column.  The equilibrium records of the 2M + 1 rows around the
+// current one (M = the stencil's largest |c|) stay in LDS, packed (8 bytes
+// each): 2M + 2 rotating row slots (4 for M = 1, 8 for M = 2, 3) of
+// blockDim.x + 2M records (the block's columns and M halo columns on each
+// side).  Every thread loads the record of its own column for the row ahead
+// and one barrier per row publishes it, so a node loads one record instead of
+// one per pulled population.  The blocks of the last grid column update the
+// nodes within M of the lattice edge instead (general pull: wrap + boundary
+// conditions), a share per row of blocks, so the edge runs alongside the
+// interior rather than as a separate latency-bound launch.  Measured on
+// MI355X (D2Q9 4096^2, FP16-NEQ): 8 rows, 204 us per step, against 305 us
+// with one node per thread gathering every neighbour's record.
 template <int QID, int COLL, class ST, int R, bool NT>
 __global__ void __launch_bounds__(256)
 k_fused_neq_rows(const typename ST::store_t *__restrict__ C,
-                     typename ST::store_t *__restrict__ Cn, const StepParams<float> p) {
-    static_assert(QT<QID>::MAXC == 1, "row marching keeps a 3-row record window");
+                 typename ST::store_t *__restrict__ Cn, const StepParams<float> p) {
     constexpr int Q = QT<QID>::Q;
+    constexpr int M = QT<QID>::MAXC;
+    constexpr int SL = M == 1 ? 4 : 8;  // row slots: at least 2M + 2, a power of two
+    static_assert(2 * M + 2 <= SL, "record window too small for the stencil");
     using S = typename ST::store_t;
-    __shared__ Rec16 win[4][256 + 2];
+    __shared__ Rec16 win[SL][256 + 2 * M];
     const int tid = threadIdx.x;
     if ((int)blockIdx.x == (int)gridDim.x - 1) {
         // the last block column: this row of blocks' share of the edge nodes
-        const long long ne = edge_count(p.nx, p.ny);
+        const long long ne = edge_count(p.nx, p.ny, M);
         const long long chunk = (ne + gridDim.y - 1) / gridDim.y;
         const long long k1 = min(ne, (long long)(blockIdx.y + 1) * chunk);
         for (long long k = (long long)blockIdx.y * chunk + tid; k < k1; k += blockDim.x) {
             int ex, ey;
-            if (edge_node(k, p.nx, p.ny, ex, ey)) neq_edge_update<QID, COLL, ST>(C, Cn, p, ex, ey);
+            if (edge_node(k, p.nx, p.ny, M, ex, ey)) neq_edge_update<QID, COLL, ST>(C, Cn, p, ex, ey);
         }
         return;
     }
@@ -1041,31 +1129,40 @@ k_fused_neq_rows(const typename ST::store_t *__restrict__ C,
     const int last = min((int)blockDim.x, p.nx - xb) - 1;  // last active thread
     const bool active = tid <= last;
     const Rec16 *rb = ST::recs(C);
+    // halo columns, wrapped once (the launch needs nx > 2M)
+    const int xl = xb - M + tid, xr = xb + last + 1 + tid;
+    const int xlw = xl < 0 ? xl + p.nx : xl, xrw = xr >= p.nx ? xr - p.nx : xr;
     auto load_row = [&](int yy) {
         const int yw = yy < 0 ? yy + p.ny : (yy >= p.ny ? yy - p.ny : yy);
         const Rec16 *r = rb + (long long)p.nx * yw;
-        Rec16 *w = win[yy & 3];
-        if (active) w[1 + tid] = *at32(r, x);
-        if (tid == 0) w[0] = *at32(r, xb == 0 ? p.nx - 1 : xb - 1);
-        if (tid == last) w[last + 2] = *at32(r, xb + last + 1 == p.nx ? 0 : xb + last + 1);
+        Rec16 *w = win[yy & (SL - 1)];
+        if (active) w[M + tid] = *at32(r, x);
+        if (tid < M) {
+            w[tid] = *at32(r, xlw);
+            w[M + last + 1 + tid] = *at32(r, xrw);
+        }
     };
-    load_row(y0 - 1);
-    load_row(y0);
-    const bool xin = active && (x >= 1) && (x < p.nx - 1);
+#pragma unroll
+    for (int k = -M; k < M; ++k) load_row(y0 + k);
+    const bool xin = active && (x >= M) && (x < p.nx - M);
 #pragma unroll 1
     for (int r = 0; r < R; ++r) {
         const int y = y0 + r;
         if (y >= p.ny) break;
-        load_row(y + 1);
+        load_row(y + M);
         __syncthreads();
-        if (xin && y >= 1 && y < p.ny - 1) {
+        // the plane size made opaque per row: the Q plane-row addresses are
+        // formed in scalar registers when used instead of being kept live
+        // across the loop (2Q registers for the wide stencils)
+        long long pl = p.plane;
+        asm volatile("" : "+s"(pl));
+        if (xin && y >= M && y < p.ny - M) {
             float f[Q];
             auto one = [&](auto Ic) {
                 constexpr int i = decltype(Ic)::value;
                 constexpr int cx = QT<QID>::cx(i), cy = QT<QID>::cy(i);
-                const S h = *at32(ST::pops(C, p.plane) + p.plane * i + (long long)p.nx * (y - cy),
-                                  x - cx);
-                f[i] = geq_rec<QID, i>(ST::unpack(win[(y - cy) & 3][1 + tid - cx]), p) +
+                const S h = *at32(ST::pops(C, pl) + pl * i + (long long)p.nx * (y - cy), x - cx);
+                f[i] = geq_rec<QID, i>(ST::unpack(win[(y - cy) & (SL - 1)][M + tid - cx]), p) +
                        ST::to_f(h, p);
             };
             adj_each_node<QID>(one);
@@ -1079,7 +1176,7 @@ k_fused_neq_rows(const typename ST::store_t *__restrict__ C,
                 constexpr int i = decltype(Ic)::value;
                 Pack<S, 1> pk;
                 pk.v[0] = ST::raw(nq[i]);
-                put_pack<NT>(at32(ST::pops(Cn, p.plane) + p.plane * i + (long long)p.nx * y, x), pk);
+                put_pack<NT>(at32(ST::pops(Cn, pl) + pl * i + (long long)p.nx * y, x), pk);
             };
             adj_each_node<QID>(st);
         }
@@ -1093,7 +1190,7 @@ k_fused_edge(const typename ST::store_t *__restrict__ C, typename ST::store_t *_
              const StepParams<float> p) {
     const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     int x, y;
-    if (!edge_node(k, p.nx, p.ny, x, y)) return;
+    if (!edge_node(k, p.nx, p.ny, QT<QID>::MAXC, x, y)) return;
     neq_edge_update<QID, COLL, ST>(C, Cn, p, x, y);
 }
 
@@ -1213,12 +1310,13 @@ int launch_neq_rows_r(const Launch &L, const StepParams<typename ST::T> &p, hipS
     const dim3 grid((p.nx + bx - 1) / bx, (p.ny + R - 1) / R, 1), block(bx, 1, 1);
     const S *src = static_cast<const S *>(L.src);
     S *dst = static_cast<S *>(L.dst);
-    if (p.nx > 2 && p.ny > 2) {
+    constexpr int M = QT<QID>::MAXC;
+    if (p.nx > 2 * M && p.ny > 2 * M) {
         const dim3 g2(grid.x + 1, grid.y, 1);  // + one block column for the edge
         if (L.nt) k_fused_neq_rows<QID, COLL, ST, R, true><<<g2, block, 0, s>>>(src, dst, p);
         else k_fused_neq_rows<QID, COLL, ST, R, false><<<g2, block, 0, s>>>(src, dst, p);
     } else {
-        const long long nedge = edge_count(p.nx, p.ny);
+        const long long nedge = edge_count(p.nx, p.ny, M);
         k_fused_edge<QID, COLL, ST><<<(unsigned)((nedge + 255) / 256), 256, 0, s>>>(src, dst, p);
     }
     return hipGetLastError() == hipSuccess ? 0 : -4;
@@ -1228,7 +1326,7 @@ template <int QID, int COLL, class ST>
 int launch_fused_c(const Launch &L, const StepParams<typename ST::T> &p, hipStream_t s) {
     constexpr int MV = max_vec<QID, ST>();
     const int vec = L.vec < 1 ? 1 : L.vec;
-    if constexpr (ST::kNeq && QT<QID>::MAXC == 1) {
+    if constexpr (ST::kNeq) {
         if (vec > 1) {
             if (p.nx >= (1 << 27)) return -5;
             switch (vec) {
@@ -1267,10 +1365,18 @@ int launch_op(const Launch &L, const StepParams<typename ST::T> &p, hipStream_t 
     case Op::Collide:
         if (L.coll == 0) k_collide<QID, 0, ST><<<grid, block, 0, s>>>(src, dst, p);
         else if (L.coll == 1) k_collide<QID, 1, ST><<<grid, block, 0, s>>>(src, dst, p);
-        else if constexpr (ST::id == 0) k_collide<QID, 2, ST><<<grid, block, 0, s>>>(src, dst, p);
-        else return -2;
+        else if (L.coll == 2) {
+            if constexpr (ST::id == 0) k_collide<QID, 2, ST><<<grid, block, 0, s>>>(src, dst, p);
+            else return -2;
+        } else if (L.coll == 3) {
+            if constexpr (ST::id == 0) k_collide<QID, 3, ST><<<grid, block, 0, s>>>(src, dst, p);
+            else return -2;
+        } else {
+            return -2;
+        }
         break;
     case Op::Fused:
+        if (L.coll == 3) return -2;  // collide-only model: the host runs collide + stream
         if (L.coll == 2) {
             if constexpr (ST::id == 0) k_fused<QID, 2, ST><<<grid, block, 0, s>>>(src, dst, p);
             else return -2;

@@ -10,9 +10,10 @@ from ._lib import LbmError, lib  # noqa: F401
 from .adjoint import AdjointProblem, divergence_free_field, rrule, simulate_loss  # noqa: F401
 from .batch import LatticeBatch  # noqa: F401
 from .boundary_conditions import BounceBack, East, MovingWall, North, South, West  # noqa: F401
-from .collision_models import (MRT, SRT, TRT, TRT_Λ, collision_model_from_viscosity,  # noqa: F401
-                               trt_magic)
-from .initial_conditions import cell_centres, equilibrium_populations  # noqa: F401
+from .collision_models import (MRT, SRT, TRT, TRT_Λ, IterativeInitializationCollisionModel,  # noqa: F401
+                               collision_model_from_viscosity, trt_magic)
+from .initial_conditions import (IterativeInitializationMeiEtAl, cell_centres,  # noqa: F401
+                                 equilibrium_populations, initialize_mei)
 from .lattice import HipLattice  # noqa: F401
 from .model import (LatticeBoltzmannModel, ProcessingMethod, apply_boundary_conditions_,  # noqa: F401
                     autotune, collide_model_, next_, simulate, simulate_timed, stream_model_)

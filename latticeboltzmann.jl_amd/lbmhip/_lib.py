@@ -15,7 +15,7 @@ LIB_PATH = os.environ.get("LBM_HIP_LIB",
 STORE_F64, STORE_F32, STORE_F16DEV, STORE_BF16DEV, STORE_F16NEQ, STORE_BF16NEQ = 0, 1, 2, 3, 4, 5
 STORAGES = {"f64": STORE_F64, "f32": STORE_F32, "f16dev": STORE_F16DEV, "bf16dev": STORE_BF16DEV,
             "f16neq": STORE_F16NEQ, "bf16neq": STORE_BF16NEQ}
-SRT_KIND, TRT_KIND, MRT_KIND = 0, 1, 2
+SRT_KIND, TRT_KIND, MRT_KIND, MEI_KIND = 0, 1, 2, 3
 FORCE_NONE, FORCE_CONST, FORCE_FIELD = 0, 1, 2
 BOUNCE_BACK, MOVING_WALL = 0, 1
 NORTH, EAST, SOUTH, WEST = 0, 1, 2, 3
@@ -37,6 +37,7 @@ EXPORTS = [
     "lbm_adjoint_gradient",
     "lbm_expected_create", "lbm_expected_destroy", "lbm_expected_upload", "lbm_track_errors",
     "lbm_velocity_tracker_create", "lbm_velocity_tracker_destroy", "lbm_velocity_stats",
+    "lbm_model_set_mei", "lbm_field_node", "lbm_mei_initialize",
 ]
 
 
@@ -125,6 +126,9 @@ def lib() -> ctypes.CDLL:
     L.lbm_adjoint_info.argtypes = [vp, ip, ip, ctypes.POINTER(i64)]
     L.lbm_adjoint_loss.argtypes = [vp, dp, dp, dp, dp]
     L.lbm_adjoint_gradient.argtypes = [vp, dp, dp, dp, dp, dp, dp]
+    L.lbm_model_set_mei.argtypes = [vp, d, dp, i, i]
+    L.lbm_field_node.argtypes = [vp, i, i, dp]
+    L.lbm_mei_initialize.argtypes = [vp, vp, vp, d, i64, ctypes.POINTER(i64)]
     L.lbm_expected_create.argtypes = [ctypes.POINTER(vp), i, i]
     L.lbm_expected_destroy.argtypes = [vp]
     L.lbm_expected_upload.argtypes = [vp, dp]

@@ -93,6 +93,27 @@ class MRT(CollisionModel):
         self.force = force
 
 
+class IterativeInitializationCollisionModel(CollisionModel):
+    """IterativeInitializationCollisionModel(q, τ, problem)
+    (collision_models/iterative_initialization.jl:1-40): relaxes towards
+    w_i ρ + N_i, where N_i, the nonlinear part of the equilibrium of the
+    prescribed lattice velocity, is formed once (on the host, in the
+    reference's order) and kept on the device.  ``u`` is that velocity,
+    (nx, ny, 2); ``IterativeInitializationCollisionModel.for_problem(q, τ,
+    problem)`` takes it from ``problem.lattice_velocity_field()``."""
+
+    def __init__(self, τ: float, u):
+        self.τ = float(τ)
+        self.u = np.array(u, dtype=np.float64, order="F", copy=True)
+        if self.u.ndim != 3 or self.u.shape[2] != 2:
+            raise ValueError(f"lattice velocity must be (nx, ny, 2), got {self.u.shape}")
+        self.force = None
+
+    @classmethod
+    def for_problem(cls, q: Quadrature, τ: float, problem):
+        return cls(τ, problem.lattice_velocity_field())
+
+
 def collision_model_from_viscosity(kind, q: Quadrature, ν: float, force=None, Λ: float = 0.25):
     """CollisionModel(SRT|TRT|MRT, q, problem) with τ = s ν + 1/2
     (collision_models.jl:11-18, srt.jl:7-16, trt.jl:8-21, mrt.jl:36-47)."""

@@ -41,3 +41,40 @@ def cell_centres(n: int, length: float = 1.0) -> np.ndarray:
     """range(Δx/2, stop = L - Δx/2, length = n) -- problems.jl:18-26"""
     dx = length / n
     return np.linspace(dx / 2, length - dx / 2, n)
+
+
+class IterativeInitializationMeiEtAl:
+    """IterativeInitializationMeiEtAl(τ, ϵ) (initial_conditions/mei_et_al.jl:
+    1-10); IterativeInitialization() = (1.0, 1e-7)."""
+
+    def __init__(self, τ: float = 1.0, ϵ: float = 1e-7):
+        self.τ = float(τ)
+        self.ϵ = float(ϵ)
+
+
+def initialize_mei(strategy: IterativeInitializationMeiEtAl, q: Quadrature, problem,
+                   max_steps: int = 10000):
+    """initialize(::IterativeInitializationMeiEtAl, q, problem) (mei_et_al.jl:
+    13-41) on the device: f = w everywhere, then the collide-only model of the
+    prescribed velocity, stream! and the problem's boundary conditions, until
+    the density at node (nx, ny) changes by less than ϵ (or more than 100)
+    between steps, at most max_steps (the reference's 1:10000).  ``problem``
+    gives ``NX``, ``NY``, ``lattice_velocity_field()`` and optionally
+    ``boundary_conditions()``.  Returns (f_stream as a HipLattice, steps)."""
+    import ctypes
+
+    from . import _lib
+    from .collision_models import IterativeInitializationCollisionModel
+    from .lattice import HipLattice
+    from .operators import native_model
+
+    cm = IterativeInitializationCollisionModel.for_problem(q, strategy.τ, problem)
+    bcs = list(problem.boundary_conditions()) if hasattr(problem, "boundary_conditions") else []
+    fs = HipLattice(q, problem.NX, problem.NY)
+    fc = HipLattice(q, problem.NX, problem.NY)
+    m = native_model(q).configure(cm, fs, bcs)
+    steps = ctypes.c_int64()
+    _lib.check(_lib.lib().lbm_mei_initialize(m.handle, fs.handle, fc.handle, strategy.ϵ,
+                                             int(max_steps), ctypes.byref(steps)),
+               "lbm_mei_initialize")
+    return fs, int(steps.value)

@@ -54,6 +54,13 @@ class HipLattice:
         _lib.check(_lib.lib().lbm_field_upload(self._h, _lib.dptr(a)), "lbm_field_upload")
         return self
 
+    def node(self, x: int, y: int) -> np.ndarray:
+        """f[x, y, :] with 1-based (x, y), read from the device in FP64"""
+        out = np.empty(self.quadrature.Q)
+        _lib.check(_lib.lib().lbm_field_node(self._h, int(x), int(y), _lib.dptr(out)),
+                   "lbm_field_node")
+        return out
+
     def to_numpy(self) -> np.ndarray:
         out = np.empty(self.shape, dtype=np.float64, order="F")
         _lib.check(_lib.lib().lbm_field_download(self._h, _lib.dptr(out)), "lbm_field_download")

@@ -21,7 +21,7 @@ import numpy as np
 
 from . import _lib
 from .boundary_conditions import BoundaryCondition
-from .collision_models import MRT, SRT, TRT, CollisionModel
+from .collision_models import MRT, SRT, TRT, CollisionModel, IterativeInitializationCollisionModel
 from .lattice import HipLattice
 from .quadratures import Quadrature
 
@@ -59,6 +59,15 @@ class NativeModel:
             if key != self._coll_key:
                 t = np.ascontiguousarray(cm.τs, dtype=np.float64)
                 _lib.check(L.lbm_model_set_mrt(self._h, _lib.dptr(t), t.size), "lbm_model_set_mrt")
+        elif isinstance(cm, IterativeInitializationCollisionModel):
+            # keyed by the model object, which the handle keeps alive (its
+            # velocity is a private copy, fixed at construction)
+            key = ("mei", id(cm))
+            self._coll_obj = cm
+            if key != self._coll_key:
+                nx, ny, _ = cm.u.shape
+                _lib.check(L.lbm_model_set_mei(self._h, cm.τ, _lib.dptr(cm.u), nx, ny),
+                           "lbm_model_set_mei")
         else:
             raise TypeError(f"unsupported collision model {cm!r}")
         self._coll_key = key

@@ -142,6 +142,11 @@ class TaylorGreenVortex:
         ux, uy = self.velocity(0.0)
         return equilibrium_populations(q, rho, self.u_max * ux, self.u_max * uy)
 
+    def lattice_velocity_field(self) -> np.ndarray:
+        """lattice_velocity = u_max velocity(x, y, 0) (problems.jl:91-92), (NX, NY, 2)"""
+        ux, uy = self.velocity(0.0)
+        return np.stack([self.u_max * ux, self.u_max * uy], axis=-1)
+
     def lattice_force(self):
         """u_max Δt force(x, y) = u_max Δt 2 ν_dimensionless u(x, y, 0) when
         static (taylor_green_vortex.jl:110-114, problems.jl:98-99), as

@@ -210,11 +210,13 @@ class DensityConvergence(StopCriteria):
         self.rho = np.zeros((nx, ny))
 
     def should_stop_(self, q: Quadrature, f: HipLattice) -> bool:
-        from .moments import density
-
         nx, ny = self.rho_old.shape
         self.rho_old[nx - 1, ny - 1] = self.rho[nx - 1, ny - 1]
-        self.rho[nx - 1, ny - 1] = density(f)[nx - 1, ny - 1]
+        pops = f.node(nx, ny)
+        r = pops[0] + pops[1]
+        for v in pops[2:]:
+            r = r + v
+        self.rho[nx - 1, ny - 1] = r
         d = float(np.linalg.norm(self.rho - self.rho_old))
         return d < self.eps or d > 100.0
 

@@ -88,6 +88,10 @@ def lib() -> ctypes.CDLL:
         _lib.lbo_moments.argtypes = [qp, i, i, dp, d, dp, dp, dp, dp, dp, dp]
         _lib.lbo_track_errors.argtypes = [qp, i, i, dp, dp, dp, d, d, d, dp]
         _lib.lbo_velocity_stats.argtypes = [qp, i, i, dp, dp, dp]
+        _lib.lbo_mei_nonlinear.argtypes = [qp, i, i, dp, dp]
+        _lib.lbo_collide_mei.argtypes = [qp, i, i, dp, dp, d, dp]
+        _lib.lbo_mei_initialize.argtypes = [qp, i, i, dp, d, d, ctypes.c_long, bp, i, dp, dp]
+        _lib.lbo_mei_initialize.restype = ctypes.c_long
         _lib.lbo_set_threads.argtypes = [i]
         _lib.lbo_get_threads.restype = i
     return _lib
@@ -285,3 +289,39 @@ def velocity_stats(qid, f, u_old=None):
     lib().lbo_velocity_stats(ctypes.byref(q), nx, ny, _dp(f),
                              None if u_old is None else _dp(u_old), _dp(out))
     return out
+
+
+def mei_nonlinear(qid, u):
+    """nonlinear_term of IterativeInitializationCollisionModel
+    (iterative_initialization.jl:12-40) for the lattice velocity u (nx, ny, 2)."""
+    q = quadrature(qid)
+    u = np.asfortranarray(u, dtype=np.float64)
+    nx, ny, _ = u.shape
+    N = np.zeros((nx, ny, q.Q), order="F")
+    lib().lbo_mei_nonlinear(ctypes.byref(q), nx, ny, _dp(u), _dp(N))
+    return N
+
+
+def collide_mei(qid, fin, fout, tau, N):
+    q = quadrature(qid)
+    _check_f(fin, q.Q)
+    _check_f(fout, q.Q)
+    nx, ny, _ = fin.shape
+    N = np.asfortranarray(N, dtype=np.float64)
+    lib().lbo_collide_mei(ctypes.byref(q), nx, ny, _dp(fin), _dp(fout), float(tau), _dp(N))
+
+
+def mei_initialize(qid, u, tau=1.0, eps=1e-7, max_steps=10000, bcs=()):
+    """initialize(IterativeInitializationMeiEtAl(tau, eps), q, problem)
+    (mei_et_al.jl:13-41): returns (f_stream, f_collision, steps run)."""
+    q = quadrature(qid)
+    u = np.asfortranarray(u, dtype=np.float64)
+    nx, ny, _ = u.shape
+    fs = np.zeros((nx, ny, q.Q), order="F")
+    fc = np.zeros((nx, ny, q.Q), order="F")
+    arr = make_bcs(list(bcs))
+    steps = lib().lbo_mei_initialize(ctypes.byref(q), nx, ny, _dp(u), float(tau), float(eps),
+                                     int(max_steps), arr, len(bcs), _dp(fs), _dp(fc))
+    if steps < 0:
+        raise IndexError("boundary range outside the lattice (BoundsError in the reference)")
+    return fs, fc, int(steps)

@@ -363,3 +363,17 @@ def velocity_stats(q, f, u_old=None):
         out[2] = float(np.sum((ux - u_old[..., 0]) ** 2 + (uy - u_old[..., 1]) ** 2))
         out[3] = float(np.sum(u_old ** 2))
     return np.array(out)
+
+
+def mei_nonlinear(q, u):
+    """The nonlinear equilibrium term of the iterative initialization
+    (iterative_initialization.jl:12-40) as tensor algebra:
+    w_i (s c.u + (s^2 (c.u)^2 - s |u|^2) / 2)."""
+    cu = np.einsum("xyd,di->xyi", np.asarray(u, dtype=np.float64), q.c.astype(float))
+    usq = np.sum(np.asarray(u) ** 2, axis=2)[..., None]
+    return q.w * (q.s * cu + 0.5 * (q.s ** 2 * cu ** 2 - q.s * usq))
+
+
+def collide_mei(q, f, tau, N):
+    rho = f.sum(axis=2)[..., None]
+    return (1.0 - 1.0 / tau) * f + (1.0 / tau) * (q.w * rho + N)

@@ -743,3 +743,87 @@ void lbo_velocity_stats(const lbo_quad *q, int nx, int ny, const double *fin, do
     out[3] = old2;
     out[4] = nonfinite;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Iterative initialization of Mei et al.                                   */
+/* (initial_conditions/mei_et_al.jl:13-41,                                  */
+/* collision_models/iterative_initialization.jl:12-58,                      */
+/* processing_methods/process_iterative_initialization.jl:1-22)             */
+/* ------------------------------------------------------------------------ */
+/* nonlinear_term[x, y, i] = w_i rho_0 (a_H_1 + a_H_2 / 2) of the lattice      */
+/* velocity u (nx, ny, 2) (iterative_initialization.jl:12-40)                */
+void lbo_mei_nonlinear(const lbo_quad *q, int nx, int ny, const double *u, double *N) {
+    const int Q = q->Q;
+    const size_t plane = (size_t)nx * (size_t)ny;
+    const double cs = q->s;
+    for (int x = 0; x < nx; ++x)
+        for (int y = 0; y < ny; ++y) {
+            const size_t n = (size_t)x + (size_t)nx * (size_t)y;
+            const double u0 = u[n], u1 = u[plane + n];
+            const double usq = u0 * u0 + u1 * u1;
+            for (int i = 0; i < Q; ++i) {
+                const double ud = (0.0 + (double)q->cx[i] * u0) + (double)q->cy[i] * u1;
+                const double a1 = cs * ud;
+                const double a2 = (cs * cs) * (ud * ud) - cs * usq;
+                N[IDX(x, y, i)] = (q->w[i] * 1.0) * (a1 + a2 / 2.0);
+            }
+        }
+}
+
+/* collide!(::IterativeInitializationCollisionModel) (iterative_initialization.jl:41-58):
+ * f_out = (1 - 1/tau) f_in + (1/tau) (w_i rho + N_i) */
+void lbo_collide_mei(const lbo_quad *q, int nx, int ny, const double *fin, double *fout,
+                     double tau, const double *N) {
+    const int Q = q->Q;
+#pragma omp parallel for collapse(2) num_threads(g_threads) schedule(static)
+    for (int y = 0; y < ny; ++y)
+        for (int x = 0; x < nx; ++x) {
+            double f[LBO_QMAX];
+            for (int i = 0; i < Q; ++i) f[i] = fin[IDX(x, y, i)];
+            const double rho = lbo_density(q, f);
+            for (int i = 0; i < Q; ++i) {
+                const double feq = q->w[i] * rho + N[IDX(x, y, i)];
+                fout[IDX(x, y, i)] = (1.0 - 1.0 / tau) * f[i] + (1.0 / tau) * feq;
+            }
+        }
+}
+
+/* initialize(::IterativeInitializationMeiEtAl, q, problem) (mei_et_al.jl:13-41):
+ * f = w everywhere, then simulate(model, 1:max_steps) with the collision above,
+ * stream! and the BC list; after each step next!(ProcessIterativeInitialization)
+ * checks DensityConvergence (density_convergence.jl:6-17), whose loop visits
+ * only node (nx, ny): stop when |rho - rho_old| there is < eps or > 100.
+ * Returns the number of steps run; f_stream holds the result. */
+long lbo_mei_initialize(const lbo_quad *q, int nx, int ny, const double *u, double tau,
+                        double eps, long max_steps, const lbo_bc *bcs, int nbc,
+                        double *f_stream, double *f_collision) {
+    const int Q = q->Q;
+    const size_t plane = (size_t)nx * (size_t)ny;
+    double *N = (double *)malloc(plane * (size_t)Q * sizeof(double));
+    if (!N) return -1;
+    lbo_mei_nonlinear(q, nx, ny, u, N);
+    for (int i = 0; i < Q; ++i)
+        for (size_t n = 0; n < plane; ++n) {
+            f_stream[plane * (size_t)i + n] = q->w[i];
+            f_collision[plane * (size_t)i + n] = q->w[i];
+        }
+    double rho = 0.0, rho_old = 0.0;
+    const size_t corner = (size_t)(nx - 1) + (size_t)nx * (size_t)(ny - 1);
+    long t = 1;
+    for (; t <= max_steps; ++t) {
+        lbo_collide_mei(q, nx, ny, f_stream, f_collision, tau, N);
+        lbo_stream(q, nx, ny, f_collision, f_stream);
+        if (lbo_apply_bcs(q, nx, ny, bcs, nbc, f_stream, f_collision) != 0) {
+            free(N);
+            return -1;
+        }
+        double f[LBO_QMAX];
+        for (int i = 0; i < Q; ++i) f[i] = f_stream[plane * (size_t)i + corner];
+        rho_old = rho;
+        rho = lbo_density(q, f);
+        const double d = fabs(rho - rho_old);
+        if (d < eps || d > 100.0) break;
+    }
+    free(N);
+    return t > max_steps ? max_steps : t;
+}

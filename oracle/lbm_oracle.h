@@ -85,6 +85,16 @@ void lbo_track_errors(const lbo_quad *q, int nx, int ny, const double *f,
 void lbo_velocity_stats(const lbo_quad *q, int nx, int ny, const double *f, double *u_old,
                         double *out);
 
+/* iterative initialization of Mei et al.: the nonlinear term of the lattice
+ * velocity u (nx, ny, 2); its collision; the whole initialization (returns the
+ * steps run, -1 on a bad BC range) */
+void lbo_mei_nonlinear(const lbo_quad *q, int nx, int ny, const double *u, double *N);
+void lbo_collide_mei(const lbo_quad *q, int nx, int ny, const double *fin, double *fout,
+                     double tau, const double *N);
+long lbo_mei_initialize(const lbo_quad *q, int nx, int ny, const double *u, double tau,
+                        double eps, long max_steps, const lbo_bc *bcs, int nbc,
+                        double *f_stream, double *f_collision);
+
 /* OpenMP thread count used by the node loops (1 = scalar) */
 void lbo_set_threads(int n);
 int lbo_get_threads(void);

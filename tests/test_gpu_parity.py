@@ -469,3 +469,42 @@ def test_neq_row_marching_matches_single_node(storage, rows, shape, bcname):
                 assert d.max() <= tol, (steps, d.max())
                 assert np.count_nonzero(d > 1e-6) <= frac * d.size, (steps,
                                                                      np.count_nonzero(d > 1e-6))
+
+
+@pytest.mark.parametrize("qid", [13, 17, 21, 37])
+@pytest.mark.parametrize("storage", ["f16neq", "bf16neq"])
+@pytest.mark.parametrize("rows", [2, 8])
+@pytest.mark.parametrize("shape", [(5, 5), (7, 9), (9, 7), (40, 23), (70, 31)])
+@pytest.mark.parametrize("bcname", ["periodic", "couette"])
+def test_neq_row_marching_wide_stencils(qid, storage, rows, shape, bcname):
+    """The row-marching kernel on stencils reaching 2 (D2Q13) or 3 (D2Q17/21/37)
+    nodes: an LDS window of 2M + 1 record rows and M halo columns, and an edge
+    frame M nodes wide.  Against the one node per thread kernel, with the same
+    rounding-flip allowance as for D2Q9; lattices no wider or taller than 2M
+    have no interior and take the edge kernel alone.  After 7 steps the
+    rounding flips have spread further through the wider neighbourhoods
+    (measured: 1.2e-4 w at most, D2Q17 FP16-NEQ 70 x 31), so the allowance is
+    three times the one-step bound there; a wrong window row or halo column
+    moves populations by ~1e-2 w."""
+    L = _L()
+    q = _q(qid)
+    nx, ny = shape
+    f0 = random_populations(qid, nx, ny, seed=37, noise=1e-4)
+    cm = L.TRT.fields(0.7, 1.3, force_for("const", nx, ny))
+    bcs = to_lbmhip(bc_dicts(bcname, nx, ny))
+    ulp_tol = {"f16neq": 1e-4, "bf16neq": 1e-3}[storage]
+    w = np.asarray(q.weights, dtype=float)
+    w = np.where(w > 0, w, 1.0)                 # D2Q21's four zero-weight planes stay 0
+    for steps, tol in ((1, ulp_tol), (7, 3 * ulp_tol)):
+        out = []
+        for launch in ((64, 1, 1, 1), (64, 1, rows, 1), (256, 1, rows, 1)):
+            m = L.LatticeBoltzmannModel.from_populations(q, f0, cm, bcs, storage=storage)
+            m.launch = launch
+            L.simulate(m, range(steps))
+            out.append((m.f_stream.to_numpy(), m.f_collision.to_numpy()))
+        for other in out[1:]:
+            for a, b in zip(out[0], other):
+                d = np.abs((a - b) / w)
+                assert d.max() <= tol, (steps, d.max())
+                assert np.count_nonzero(d > 1e-6) <= 0.1 * d.size, (steps,
+                                                                    np.count_nonzero(d > 1e-6))

@@ -132,10 +132,14 @@ def test_processing_errors():
 
 
 def test_track_hydrodynamic_errors_over_a_decaying_vortex(oracle_lib):
-    """simulate(model, 0:250) with TrackHydrodynamicErrors on the GPU: the rows
-    at steps 100, 200 and 250 equal the oracle's rows on the oracle's own FP64
-    run (bit-exact populations, so only the summation order differs), and the
-    populations equal a run without processing."""
+    """simulate(model, 0:250) with TrackHydrodynamicErrors on the GPU records
+    the row of step 250 (track_hydrodynamic_errors.jl:62-66: without
+    should_process only the last step and a stop record one); it equals the
+    oracle's row on the oracle's own FP64 run (bit-exact populations, so only
+    the summation order differs), and the populations equal a run without
+    processing.  With should_process every step records a row, the last one
+    twice (the loop's next! and the one after it, lattice_boltzmann_model.jl:
+    69-74)."""
     L = _L()
     pb = L.TaylorGreenVortex(scale=2, static=False)      # 32 x 32, ν = 1/6
     q = L.D2Q9()
@@ -145,18 +149,25 @@ def test_track_hydrodynamic_errors_over_a_decaying_vortex(oracle_lib):
     m = L.LatticeBoltzmannModel.from_populations(q, f0, L.SRT(tau), processing_method=pm,
                                                  delta_t=pb.delta_t())
     L.simulate(m, range(0, 251))
-    assert [r.timestep for r in pm.df] == [100, 200, 250]
+    assert [r.timestep for r in pm.df] == [250]
     m2 = L.LatticeBoltzmannModel.from_populations(q, f0, L.SRT(tau))
     L.simulate(m2, range(0, 251))
     assert np.array_equal(m.f_stream.to_numpy(), m2.f_stream.to_numpy())
-    # the oracle's run and rows
-    fs = np.asfortranarray(f0.copy())
-    fc = np.asfortranarray(f0.copy())
-    done = 0
+
+    pm3 = L.TrackHydrodynamicErrors(pb, should_process=True, n_steps=10 ** 6)
+    m3 = L.LatticeBoltzmannModel.from_populations(q, f0, L.SRT(tau), processing_method=pm3,
+                                                  delta_t=pb.delta_t())
+    L.simulate(m3, range(0, 5))
+    assert [r.timestep for r in pm3.df] == [1, 2, 3, 4, 5, 5]
+    assert pm3.df[-1] == pm3.df[-2]
+
     planes = np.asfortranarray(pb.expected_planes(q))
     x, y = pb.range()
     delta = ((x[-1] - x[0]) / (len(x) - 1)) * ((y[-1] - y[0]) / (len(y) - 1))
-    for row in pm.df:
+    fs = np.asfortranarray(f0.copy())
+    fc = np.asfortranarray(f0.copy())
+    done = 0
+    for row in pm3.df[:5] + pm.df:
         oracle_lib.simulate(9, fs, fc, 0, [tau], row.timestep - done)
         done = row.timestep
         aff = pb.expected_affine(q, row.timestep * pb.delta_t())
@@ -164,4 +175,4 @@ def test_track_hydrodynamic_errors_over_a_decaying_vortex(oracle_lib):
                                       pb.u_max, delta)
         _check_track(np.array(list(row[1:]) + [0.0]), ref, f"t={row.timestep}")
     # the lattice solution tracks the analytic vortex: small velocity error
-    assert all(r.error_u < 2e-2 for r in pm.df)
+    assert all(r.error_u < 2e-2 for r in pm.df + pm3.df)
