@@ -343,15 +343,18 @@ void auto_launch(const lbm_model_s *m, const lbm_field_s *f, int &block, int &ro
     // 8-byte populations: 128 x 2 blocks (2-3% faster than 256 x 1 for D2Q9 and
     // D2Q17 FP64 on MI355X); everything else 256 x 1
     const bool f64 = storage_bytes(f->storage) == 8 && m->coll != LBM_MRT;
-    // 16-bit populations on the 21- and 37-velocity stencils: 64-thread blocks
+    // 16-bit deviations on the 21- and 37-velocity stencils: 64-thread blocks
     // (D2Q37 FP16-DEV 8192 x 2048: 508 us against 534 us with 256)
-    const bool wide16 = storage_bytes(f->storage) == 2 && f->Q > 17;
+    const bool wide16 = storage_bytes(f->storage) == 2 && f->Q > 17 && !rows_mode(f->storage);
     block = m->launch_block > 0 ? m->launch_block : (f64 ? 128 : (wide16 ? 64 : 256));
     rows = m->launch_rows > 0 ? m->launch_rows : (f64 ? 2 : 1);
     if (block * rows > 256) rows = 256 / block;
     vec = m->launch_vec;
     nt = m->launch_nt;
-    if (vec <= 0 && rows_mode(f->storage) && m->coll != LBM_MRT) vec = 8;
+    // non-equilibrium storages: 8 rows per thread, 2 on the 21- and
+    // 37-velocity stencils (D2Q37 FP16-NEQ 8192 x 2048: 1019 us against
+    // 1131 us with 8; D2Q9 4096^2: 197-199 us with 4 or 8, 215 us with 2)
+    if (vec <= 0 && rows_mode(f->storage) && m->coll != LBM_MRT) vec = f->Q > 17 ? 2 : 8;
     if (vec <= 0) {
         vec = 1;
         const bool neq = f->storage == LBM_STORE_F16NEQ || f->storage == LBM_STORE_BF16NEQ;

@@ -21,7 +21,7 @@ using LatticeBoltzmann: Quadrature, CollisionModel, SRT, TRT, MRT, BoundaryCondi
 import LatticeBoltzmann: collide!, stream!, apply!, simulate
 import ChainRulesCore
 
-export HipLattice, simulate_loss, adjoint_gradient
+export HipLattice, simulate_loss, adjoint_gradient, initialize_on_device
 
 const libpath = get(ENV, "LBMHIP_LIB", joinpath(@__DIR__, "..", "lib", "liblbm_hip.so"))
 
@@ -234,6 +234,34 @@ function simulate(model::LatticeBoltzmannModel{<:Quadrature, <:HipLattice}, time
         end
     end
     model
+end
+
+# ---- iterative initialization (Mei et al.) ----------------------------------
+# initialize(::IterativeInitializationMeiEtAl, q, problem) (mei_et_al.jl:13-41)
+# on the device: the collide-only model of the prescribed lattice velocity
+# (lbm_model_set_mei), stream! and the problem's BCs, stopping on the
+# DensityConvergence check of node (nx, ny); returns the HipLattice f_stream.
+function initialize_on_device(strategy::LatticeBoltzmann.IterativeInitializationMeiEtAl,
+                              q::Quadrature, problem; max_steps = 10000)
+    nx, ny = problem.NX, problem.NY
+    x_range, y_range = LatticeBoltzmann.range(problem)
+    u = Array{Float64}(undef, nx, ny, 2)
+    for x_idx in 1:nx, y_idx in 1:ny
+        v = LatticeBoltzmann.lattice_velocity(q, problem, x_range[x_idx], y_range[y_idx])
+        u[x_idx, y_idx, 1], u[x_idx, y_idx, 2] = v[1], v[2]
+    end
+    m = native_model(q)
+    GC.@preserve u check(ccall((:lbm_model_set_mei, libpath), Cint,
+                               (Ptr{Cvoid}, Cdouble, Ptr{Cdouble}, Cint, Cint),
+                               m.handle, strategy.τ, u, nx, ny), "lbm_model_set_mei")
+    set_bcs!(m, LatticeBoltzmann.boundary_conditions(problem))
+    fs, fc = HipLattice(q, nx, ny), HipLattice(q, nx, ny)
+    steps = Ref{Int64}(0)
+    check(ccall((:lbm_mei_initialize, libpath), Cint,
+                (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Cdouble, Int64, Ptr{Int64}),
+                m.handle, fs.handle, fc.handle, strategy.ϵ, max_steps, steps),
+          "lbm_mei_initialize")
+    fs
 end
 
 # ---- on-device processing ---------------------------------------------------

@@ -12,8 +12,9 @@ from .batch import LatticeBatch  # noqa: F401
 from .boundary_conditions import BounceBack, East, MovingWall, North, South, West  # noqa: F401
 from .collision_models import (MRT, SRT, TRT, TRT_Λ, IterativeInitializationCollisionModel,  # noqa: F401
                                collision_model_from_viscosity, trt_magic)
-from .initial_conditions import (IterativeInitializationMeiEtAl, cell_centres,  # noqa: F401
-                                 equilibrium_populations, initialize_mei)
+from .initial_conditions import (AnalyticalEquilibrium, IterativeInitializationMeiEtAl,  # noqa: F401
+                                 cell_centres, equilibrium_populations, initialize,
+                                 initialize_mei)
 from .lattice import HipLattice  # noqa: F401
 from .model import (LatticeBoltzmannModel, ProcessingMethod, apply_boundary_conditions_,  # noqa: F401
                     autotune, collide_model_, next_, simulate, simulate_timed, stream_model_)

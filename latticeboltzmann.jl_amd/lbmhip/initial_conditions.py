@@ -78,3 +78,24 @@ def initialize_mei(strategy: IterativeInitializationMeiEtAl, q: Quadrature, prob
                                              int(max_steps), ctypes.byref(steps)),
                "lbm_mei_initialize")
     return fs, int(steps.value)
+
+
+class AnalyticalEquilibrium:
+    """AnalyticalEquilibrium() (initial_conditions/analytical_equilibrium.jl):
+    f0 = feq(rho(x), u(x)) of the problem's analytic fields."""
+
+
+def initialize(strategy, q: Quadrature, problem, storage: str = "f64"):
+    """initialize(strategy, q, problem) (initial_conditions.jl:7-22,
+    mei_et_al.jl:13-41) returning the populations on the device: the analytic
+    equilibrium is formed on the host (``problem.initial_populations(q)``) and
+    uploaded once; the iterative initialization runs on the device."""
+    from .lattice import HipLattice
+
+    if isinstance(strategy, IterativeInitializationMeiEtAl):
+        if storage != "f64":
+            raise ValueError("the iterative initialization runs on FP64 storage only")
+        return initialize_mei(strategy, q, problem)[0]
+    if isinstance(strategy, AnalyticalEquilibrium):
+        return HipLattice.from_numpy(q, problem.initial_populations(q), storage=storage)
+    raise TypeError(f"unsupported initialization strategy {strategy!r}")

@@ -16,13 +16,10 @@ def pytest_configure(config):
 
 
 def _gpu_available() -> bool:
-    try:
-        import lbmhip
-        return lbmhip.device_count() > 0
-    except ImportError:
-        raise
-    except Exception:
-        return False
+    # a library that is missing or lacks a declared symbol raises here instead
+    # of letting every GPU test skip silently; only "no device" skips
+    import lbmhip
+    return lbmhip.device_count() > 0
 
 
 def pytest_collection_modifyitems(config, items):

@@ -78,6 +78,19 @@ def test_initialize_mei_matches_oracle(oracle_lib, eps):
     assert np.array_equal(f.to_numpy(), ref)
 
 
+def test_initialize_dispatch():
+    L = _L()
+    pb = L.TaylorGreenVortex(scale=1)                 # 16 x 16
+    q = L.D2Q9()
+    f = L.initialize(L.AnalyticalEquilibrium(), q, pb)
+    assert np.array_equal(f.to_numpy(), pb.initial_populations(q))
+    g = L.initialize(L.IterativeInitializationMeiEtAl(), q, pb)
+    h, _ = L.initialize_mei(L.IterativeInitializationMeiEtAl(), q, pb)
+    assert np.array_equal(g.to_numpy(), h.to_numpy())
+    with pytest.raises(ValueError):
+        L.initialize(L.IterativeInitializationMeiEtAl(), q, pb, storage="f32")
+
+
 def test_initialize_mei_uniform_velocity_two_steps():
     L = _L()
 
