@@ -1446,6 +1446,21 @@ k_batch(const BatchLattice *__restrict__ lats, const BatchTile *__restrict__ til
                 f[i] = ST::decode(*at32(src + p.plane * i, n), i, p);
             };
             adj_each_node<QID>(ld);
+        } else if constexpr (M == 1) {
+            // periodic pull without a branch: the wrapped neighbour columns and
+            // rows are selected per node, so the lanes of a wave that holds a
+            // lattice edge (most waves, for lattices narrower than a few
+            // hundred nodes) do not split into an interior and an edge path
+            const int xm = x == 0 ? L.nx - 1 : x - 1, xp = x == L.nx - 1 ? 0 : x + 1;
+            const int ym = y == 0 ? L.ny - 1 : y - 1, yp = y == L.ny - 1 ? 0 : y + 1;
+            auto ld = [&](auto Ic) {
+                constexpr int i = decltype(Ic)::value;
+                constexpr int cx = QT<QID>::cx(i), cy = QT<QID>::cy(i);
+                const int sx = cx == 0 ? x : (cx > 0 ? xm : xp);
+                const int sy = cy == 0 ? y : (cy > 0 ? ym : yp);
+                f[i] = ST::decode(*at32(src + p.plane * i, sx + L.nx * sy), i, p);
+            };
+            adj_each_node<QID>(ld);
         } else {
             if (x >= M && x < L.nx - M && y >= M && y < L.ny - M) {
                 auto ld = [&](auto Ic) {
