@@ -124,15 +124,18 @@ k_moments(const void *f, const StepParams<double> p, double tau, double *__restr
     if (syy_o) syy_o[n] = sg[3] - tr2;
 }
 
-// per-block FP64 partial sums: rho, rho ux, rho uy, rho |u|^2, #non-finite nodes
+// per-block FP64 partial sums: rho, rho ux, rho uy, rho |u|^2, #non-finite
+// nodes; a fixed grid strides over the lattice, and the partials are stored
+// column by column (partials[k * gridDim.x + block])
 template <int QID, int SID>
 __global__ void __launch_bounds__(256)
 k_reduce_partial(const void *f, const StepParams<double> p, double *__restrict__ partials) {
     constexpr int Q = QT<QID>::Q;
     __shared__ double red[5][256];
-    const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-    if (n < p.plane) {
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x; n < p.plane;
+         n += stride) {
         double fl[Q];
         bool finite = true;
 #pragma unroll
@@ -151,12 +154,12 @@ k_reduce_partial(const void *f, const StepParams<double> p, double *__restrict__
         for (int i = 0; i < Q; ++i) uy = uy + fl[i] * static_cast<double>(QT<QID>::cy(i));
         uy = uy / rho;
         if (finite) {
-            v[0] = rho;
-            v[1] = rho * ux;
-            v[2] = rho * uy;
-            v[3] = rho * (ux * ux + uy * uy);
+            v[0] += rho;
+            v[1] += rho * ux;
+            v[2] += rho * uy;
+            v[3] += rho * (ux * ux + uy * uy);
         } else {
-            v[4] = 1.0;
+            v[4] += 1.0;
         }
     }
     for (int k = 0; k < 5; ++k) red[k][threadIdx.x] = v[k];
@@ -167,25 +170,23 @@ k_reduce_partial(const void *f, const StepParams<double> p, double *__restrict__
         __syncthreads();
     }
     if (threadIdx.x == 0)
-        for (int k = 0; k < 5; ++k) partials[(long long)blockIdx.x * 5 + k] = red[k][0];
+        for (int k = 0; k < 5; ++k) partials[(long long)k * gridDim.x + blockIdx.x] = red[k][0];
 }
 
-// fixed-order final sum of the partials (one block): deterministic
+// fixed-order sum of each column of partials (one block per column)
 __global__ void __launch_bounds__(256)
 k_reduce_final(const double *__restrict__ partials, int npart, double *__restrict__ out) {
-    __shared__ double red[5][256];
-    double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-    for (int b = threadIdx.x; b < npart; b += blockDim.x)
-        for (int k = 0; k < 5; ++k) v[k] += partials[(long long)b * 5 + k];
-    for (int k = 0; k < 5; ++k) red[k][threadIdx.x] = v[k];
+    __shared__ double red[256];
+    const double *col = partials + (long long)blockIdx.x * npart;
+    double v = 0.0;
+    for (int b = threadIdx.x; b < npart; b += blockDim.x) v += col[b];
+    red[threadIdx.x] = v;
     __syncthreads();
     for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w)
-            for (int k = 0; k < 5; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + w];
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x == 0)
-        for (int k = 0; k < 5; ++k) out[k] = red[k][0];
+    if (threadIdx.x == 0) out[blockIdx.x] = red[0];
 }
 
 // ---------------------------------------------------------------------------
@@ -430,7 +431,7 @@ int dispatch_reduce(int qid, int storage, const void *f, const StepParams<double
     default: return -1;
     }
     if (rc) return rc;
-    k_reduce_final<<<1, 256, 0, s>>>(partials, npart, out5);
+    k_reduce_final<<<5, 256, 0, s>>>(partials, npart, out5);
     return hipGetLastError() == hipSuccess ? 0 : -4;
 }
 

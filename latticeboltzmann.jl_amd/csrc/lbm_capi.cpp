@@ -1001,7 +1001,9 @@ int lbm_reduce(lbm_field f, double *out5) {
     int fstorage;
     double *ftmp;
     if (int rc = as_f64(f, &fdata, &fstorage, &ftmp)) return rc;
-    const int npart = (int)((p.plane + 255) / 256);
+    // a fixed grid for a given lattice size: the summation order is reproducible
+    const long long blocks = (p.plane + 255) / 256;
+    const int npart = (int)(blocks < 2048 ? blocks : 2048);
     double *part = nullptr, *out = nullptr;
     HIP_TRY(hipMalloc(&part, (size_t)npart * 5 * sizeof(double)));
     hipError_t e = hipMalloc(&out, 8 * sizeof(double));
