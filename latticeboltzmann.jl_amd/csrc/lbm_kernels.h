@@ -1132,9 +1132,12 @@ k_fused_neq_rows(const typename ST::store_t *__restrict__ C,
     // halo columns, wrapped once (the launch needs nx > 2M)
     const int xl = xb - M + tid, xr = xb + last + 1 + tid;
     const int xlw = xl < 0 ? xl + p.nx : xl, xrw = xr >= p.nx ? xr - p.nx : xr;
-    auto load_row = [&](int yy) {
+    auto row_of = [&](int yy) {
         const int yw = yy < 0 ? yy + p.ny : (yy >= p.ny ? yy - p.ny : yy);
-        const Rec16 *r = rb + (long long)p.nx * yw;
+        return rb + (long long)p.nx * yw;
+    };
+    auto load_row = [&](int yy) {
+        const Rec16 *r = row_of(yy);
         Rec16 *w = win[yy & (SL - 1)];
         if (active) w[M + tid] = *at32(r, x);
         if (tid < M) {
@@ -1145,25 +1148,58 @@ k_fused_neq_rows(const typename ST::store_t *__restrict__ C,
 #pragma unroll
     for (int k = -M; k < M; ++k) load_row(y0 + k);
     const bool xin = active && (x >= M) && (x < p.nx - M);
+    // Software pipeline: the record of row y + 1 + M and the populations of
+    // row y + 1 are loaded into registers while row y is collided, and
+    // published / used one iteration later, so neither load's latency sits
+    // between a row's barrier and its arithmetic.
+    Rec16 rc{}, rl{}, rr{};
+    S h[Q];
+    auto fetch = [&](int yy, long long pl) {
+        const Rec16 *r = row_of(yy + M);
+        if (active) rc = *at32(r, x);
+        if (tid < M) {
+            rl = *at32(r, xlw);
+            rr = *at32(r, xrw);
+        }
+        if (xin && yy >= M && yy < p.ny - M) {
+            auto one = [&](auto Ic) {
+                constexpr int i = decltype(Ic)::value;
+                constexpr int cx = QT<QID>::cx(i), cy = QT<QID>::cy(i);
+                h[i] = *at32(ST::pops(C, pl) + pl * i + (long long)p.nx * (yy - cy), x - cx);
+            };
+            adj_each_node<QID>(one);
+        }
+    };
+    fetch(y0, p.plane);
 #pragma unroll 1
     for (int r = 0; r < R; ++r) {
         const int y = y0 + r;
         if (y >= p.ny) break;
-        load_row(y + M);
+        {
+            Rec16 *w = win[(y + M) & (SL - 1)];
+            if (active) w[M + tid] = rc;
+            if (tid < M) {
+                w[tid] = rl;
+                w[M + last + 1 + tid] = rr;
+            }
+        }
+        S hc[Q];
+#pragma unroll
+        for (int i = 0; i < Q; ++i) hc[i] = h[i];
         __syncthreads();
         // the plane size made opaque per row: the Q plane-row addresses are
         // formed in scalar registers when used instead of being kept live
         // across the loop (2Q registers for the wide stencils)
         long long pl = p.plane;
         asm volatile("" : "+s"(pl));
+        if (r + 1 < R && y + 1 < p.ny) fetch(y + 1, pl);
         if (xin && y >= M && y < p.ny - M) {
             float f[Q];
             auto one = [&](auto Ic) {
                 constexpr int i = decltype(Ic)::value;
                 constexpr int cx = QT<QID>::cx(i), cy = QT<QID>::cy(i);
-                const S h = *at32(ST::pops(C, pl) + pl * i + (long long)p.nx * (y - cy), x - cx);
                 f[i] = geq_rec<QID, i>(ST::unpack(win[(y - cy) & (SL - 1)][M + tid - cx]), p) +
-                       ST::to_f(h, p);
+                       ST::to_f(hc[i], p);
             };
             adj_each_node<QID>(one);
             Rec16 rec;

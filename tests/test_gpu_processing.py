@@ -176,3 +176,39 @@ def test_track_hydrodynamic_errors_over_a_decaying_vortex(oracle_lib):
         _check_track(np.array(list(row[1:]) + [0.0]), ref, f"t={row.timestep}")
     # the lattice solution tracks the analytic vortex: small velocity error
     assert all(r.error_u < 2e-2 for r in pm.df + pm3.df)
+
+
+def test_velocity_convergence_stop_matches_oracle(oracle_lib):
+    """TrackHydrodynamicErrors with a VelocityConvergenceStoppingCriteria on the
+    forced (static) vortex: the device tracker's sums give the same stop step
+    as the oracle's sequential sums of the same FP64 run (checks every 100
+    steps, stopping_criteria.jl:71-115); the row is recorded at that step."""
+    L = _L()
+    pb = L.TaylorGreenVortex(scale=1, static=True)       # 16 x 16, forced
+    q = L.D2Q9()
+    f0 = pb.initial_populations(q)
+    tau = q.speed_of_sound_squared * pb.ν + 0.5
+    F = np.asfortranarray(pb.lattice_force())
+    tol = 1e-6
+    crit = L.VelocityConvergenceStoppingCriteria(tol, nx=pb.NX, ny=pb.NY)
+    pm = L.TrackHydrodynamicErrors(pb, should_process=False, n_steps=10 ** 6, stop_criteria=crit)
+    m = L.LatticeBoltzmannModel.from_populations(q, f0, L.SRT(tau, F), processing_method=pm,
+                                                 delta_t=pb.delta_t())
+    L.simulate(m, range(0, 3000))
+    assert len(pm.df) == 1
+    t_stop = pm.df[0].timestep
+    # the oracle's run, checked at the same steps
+    S = np.asfortranarray(f0.copy())
+    C = np.asfortranarray(f0.copy())
+    u_old = np.zeros((pb.NX, pb.NY, 2), order="F")
+    t, expect = 0, None
+    while t < 3000:
+        oracle_lib.simulate(9, S, C, 0, [tau], 100, force=F)
+        t += 100
+        s = oracle_lib.velocity_stats(9, S, u_old)
+        conv = np.sqrt(s[2]) / s[3] if s[3] > 0 else np.inf
+        if conv < tol:
+            expect = t
+            break
+    assert expect == 400 and t_stop == expect
+    assert np.array_equal(m.f_stream.to_numpy(), S)
