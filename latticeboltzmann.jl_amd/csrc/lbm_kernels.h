@@ -1148,12 +1148,16 @@ k_fused_neq_rows(const typename ST::store_t *__restrict__ C,
 #pragma unroll
     for (int k = -M; k < M; ++k) load_row(y0 + k);
     const bool xin = active && (x >= M) && (x < p.nx - M);
-    // Software pipeline: the record of row y + 1 + M and the populations of
-    // row y + 1 are loaded into registers while row y is collided, and
-    // published / used one iteration later, so neither load's latency sits
-    // between a row's barrier and its arithmetic.
+    // Software pipeline (one-speed stencils): the record of row y + 1 + M and
+    // the populations of row y + 1 are loaded into registers while row y is
+    // collided, and published / used one iteration later, so neither load's
+    // latency sits between a row's barrier and its arithmetic.  D2Q9
+    // FP16-NEQ 4096^2: 194 us against 197 us; on the wide stencils the extra
+    // Q registers cost more occupancy than the overlap gains (D2Q17 385 ->
+    // 423 us), so they load in the row itself.
+    constexpr bool kPipe = M == 1;
     Rec16 rc{}, rl{}, rr{};
-    S h[Q];
+    S h[kPipe ? Q : 1];
     auto fetch = [&](int yy, long long pl) {
         const Rec16 *r = row_of(yy + M);
         if (active) rc = *at32(r, x);
@@ -1161,45 +1165,54 @@ k_fused_neq_rows(const typename ST::store_t *__restrict__ C,
             rl = *at32(r, xlw);
             rr = *at32(r, xrw);
         }
-        if (xin && yy >= M && yy < p.ny - M) {
-            auto one = [&](auto Ic) {
-                constexpr int i = decltype(Ic)::value;
-                constexpr int cx = QT<QID>::cx(i), cy = QT<QID>::cy(i);
-                h[i] = *at32(ST::pops(C, pl) + pl * i + (long long)p.nx * (yy - cy), x - cx);
-            };
-            adj_each_node<QID>(one);
+        if constexpr (kPipe) {
+            if (xin && yy >= M && yy < p.ny - M) {
+                auto one = [&](auto Ic) {
+                    constexpr int i = decltype(Ic)::value;
+                    constexpr int cx = QT<QID>::cx(i), cy = QT<QID>::cy(i);
+                    h[i] = *at32(ST::pops(C, pl) + pl * i + (long long)p.nx * (yy - cy), x - cx);
+                };
+                adj_each_node<QID>(one);
+            }
         }
     };
-    fetch(y0, p.plane);
+    if constexpr (kPipe) fetch(y0, p.plane);
 #pragma unroll 1
     for (int r = 0; r < R; ++r) {
         const int y = y0 + r;
         if (y >= p.ny) break;
-        {
+        S hc[kPipe ? Q : 1];
+        if constexpr (kPipe) {
             Rec16 *w = win[(y + M) & (SL - 1)];
             if (active) w[M + tid] = rc;
             if (tid < M) {
                 w[tid] = rl;
                 w[M + last + 1 + tid] = rr;
             }
-        }
-        S hc[Q];
 #pragma unroll
-        for (int i = 0; i < Q; ++i) hc[i] = h[i];
+            for (int i = 0; i < Q; ++i) hc[i] = h[i];
+        } else {
+            load_row(y + M);
+        }
         __syncthreads();
         // the plane size made opaque per row: the Q plane-row addresses are
         // formed in scalar registers when used instead of being kept live
         // across the loop (2Q registers for the wide stencils)
         long long pl = p.plane;
         asm volatile("" : "+s"(pl));
-        if (r + 1 < R && y + 1 < p.ny) fetch(y + 1, pl);
+        if constexpr (kPipe) {
+            if (r + 1 < R && y + 1 < p.ny) fetch(y + 1, pl);
+        }
         if (xin && y >= M && y < p.ny - M) {
             float f[Q];
             auto one = [&](auto Ic) {
                 constexpr int i = decltype(Ic)::value;
                 constexpr int cx = QT<QID>::cx(i), cy = QT<QID>::cy(i);
+                S hv;
+                if constexpr (kPipe) hv = hc[i];
+                else hv = *at32(ST::pops(C, pl) + pl * i + (long long)p.nx * (y - cy), x - cx);
                 f[i] = geq_rec<QID, i>(ST::unpack(win[(y - cy) & (SL - 1)][M + tid - cx]), p) +
-                       ST::to_f(hc[i], p);
+                       ST::to_f(hv, p);
             };
             adj_each_node<QID>(one);
             Rec16 rec;

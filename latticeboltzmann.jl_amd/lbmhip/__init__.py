@@ -21,7 +21,8 @@ from .model import (LatticeBoltzmannModel, ProcessingMethod, apply_boundary_cond
 from .moments import conserved_sums, density, moments, velocity  # noqa: F401
 from .problems import TaylorGreenVortex  # noqa: F401
 from .processing import (DensityConvergence, ExpectedFields, MeanVelocityStoppingCriteria,  # noqa: F401
-                         NoStoppingCriteria, StopCriteria, TrackHydrodynamicErrors, TrackRow,
+                         NoStoppingCriteria, StopCriteria, TakeSnapshots, TrackHydrodynamicErrors,
+                         TrackRow,
                          VelocityConvergenceStoppingCriteria, VelocityTracker,
                          stop_criteria_for, track_errors, velocity_stats)
 from .operators import apply_, collide_, stream_, stream_bc_, synchronize  # noqa: F401

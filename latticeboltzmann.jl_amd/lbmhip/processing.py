@@ -298,3 +298,36 @@ class TrackHydrodynamicErrors(ProcessingMethod):
             return False
         self.df.append(self.row(q, f, t))
         return should_stop
+
+
+# --------------------------------------------------------------------------
+# TakeSnapshots (processing_methods/take_snapshots.jl)
+# --------------------------------------------------------------------------
+class TakeSnapshots(ProcessingMethod):
+    """TakeSnapshots(problem, every_t) (take_snapshots.jl:3-29): at every
+    ``every_t``-th step (an int) or at the listed steps (a list) the
+    populations are copied.  The copies stay on the device (a device-to-device
+    copy kernel into a new HipLattice; HBM holds many of them) and are
+    downloaded only when read (:meth:`arrays`).  Plotting is not done."""
+
+    def __init__(self, problem, every_t):
+        self.problem = problem
+        self.every_t = every_t
+        self.snapshots: List[HipLattice] = []
+        self.timesteps: List[int] = []
+
+    def needs_step(self, t: int) -> bool:
+        if isinstance(self.every_t, int):
+            return t % self.every_t == 0
+        return t in self.every_t
+
+    def next_(self, q: Quadrature, f: HipLattice, t: int) -> bool:
+        if not self.needs_step(t):
+            return False
+        self.snapshots.append(f.copy())
+        self.timesteps.append(int(t))
+        return False
+
+    def arrays(self) -> List[np.ndarray]:
+        """the snapshots as (nx, ny, Q) Float64 host arrays"""
+        return [s.to_numpy() for s in self.snapshots]

@@ -212,3 +212,27 @@ def test_velocity_convergence_stop_matches_oracle(oracle_lib):
             break
     assert expect == 400 and t_stop == expect
     assert np.array_equal(m.f_stream.to_numpy(), S)
+
+
+@pytest.mark.parametrize("every", [3, [2, 7, 10]])
+def test_take_snapshots(oracle_lib, every):
+    """TakeSnapshots keeps device copies at the reference's steps
+    (take_snapshots.jl:11-29), bit-identical to the oracle's states there."""
+    L = _L()
+    q = L.D2Q9()
+    nx, ny = 20, 14
+    f0 = random_populations(9, nx, ny, seed=9)
+    pm = L.TakeSnapshots(None, every)
+    m = L.LatticeBoltzmannModel.from_populations(q, f0, L.TRT.fields(0.7, 1.2), processing_method=pm)
+    L.simulate(m, range(0, 10))                       # next! at t = 1..10, then 10 again
+    want = [t for t in range(1, 11) if (t % every == 0 if isinstance(every, int) else t in every)]
+    if 10 in want:
+        want.append(10)                               # the call after the loop
+    assert pm.timesteps == want
+    S = np.asfortranarray(f0.copy())
+    C = np.asfortranarray(f0.copy())
+    done = 0
+    for t, snap in zip(pm.timesteps, pm.arrays()):
+        oracle_lib.simulate(9, S, C, 1, [0.7, 1.2], t - done)
+        done = t
+        assert np.array_equal(snap, S), t
