@@ -162,35 +162,6 @@ int storage_bytes(int storage) {
     }
 }
 
-// Hermite tensor H_n(c) scaled by cs = 1/s (src/hermite_polynomials.jl:47-82),
-// flattened column-major.
-int delta(int a, int b) { return a == b ? 1 : 0; }
-void hermite_tensor(int n, int cx, int cy, double cs, double *H) {
-    const int xi[2] = {cx, cy};
-    const int m = 1 << n;
-    for (int k = 0; k < m; ++k) {
-        int ix[4] = {k & 1, (k >> 1) & 1, (k >> 2) & 1, (k >> 3) & 1};
-        if (n == 2) {
-            int a = ix[0], b = ix[1];
-            H[k] = (double)(xi[b] * xi[a]) - cs * (double)delta(a, b);
-        } else if (n == 3) {
-            int a = ix[0], b = ix[1], g = ix[2];
-            int A = xi[g] * xi[b] * xi[a];
-            int B = xi[a] * delta(b, g) + xi[b] * delta(a, g) + xi[g] * delta(a, b);
-            H[k] = (double)A - cs * (double)B;
-        } else {
-            int a = ix[0], b = ix[1], g = ix[2], e = ix[3];
-            int A = xi[e] * xi[g] * xi[b] * xi[a];
-            int B = xi[a] * xi[b] * delta(g, e) + xi[a] * xi[g] * delta(b, e) +
-                    xi[a] * xi[e] * delta(b, g) + xi[b] * xi[g] * delta(a, e) +
-                    xi[b] * xi[e] * delta(a, g) + xi[g] * xi[e] * delta(a, b);
-            int C = delta(a, b) * delta(g, e) + delta(a, g) * delta(b, e) +
-                    delta(a, e) * delta(b, g);
-            H[k] = ((double)A - cs * (double)B) + (cs * cs) * (double)C;
-        }
-    }
-}
-
 }  // namespace
 
 struct lbm_field_s {
@@ -213,7 +184,6 @@ struct lbm_model_s {
     std::vector<lbm_bc> bcs;
     lbm::BCDev *dB = nullptr;
     double *dMw = nullptr;
-    double *dH = nullptr;
     double *dMei = nullptr;  // iterative initialization: nonlinear term (nx, ny, Q)
     int mei_nx = 0, mei_ny = 0;
     double *dPart = nullptr;  // reduction partials
@@ -302,7 +272,6 @@ lbm::StepParams<double> make_params(const lbm_model_s *m, const lbm_field_s *f, 
             p.mrt_sn[n] = std::pow(s, (double)n);
         }
         p.tau_force = m->taus.size() > 1 ? m->taus[1] : 1.0;
-        p.mrt_H = m->dH;
     }
     p.force_kind = m->force_kind;
     p.Fx = m->F[0];
@@ -583,22 +552,10 @@ int lbm_model_create(lbm_model *out, int quadrature) {
     lbm_model m = new lbm_model_s;
     m->qid = quadrature;
     m->q = q;
-    // Hermite tables for MRT, orders 2..4 (mrt.jl:31)
-    const double cs = 1.0 / q.s;
-    std::vector<double> H((size_t)q.Q * (4 + 8 + 16), 0.0);
-    for (int n = 2; n <= 4; ++n) {
-        const int off = q.Q * (n == 2 ? 0 : (n == 3 ? 4 : 12));
-        const int mm = 1 << n;
-        for (int i = 0; i < q.Q; ++i) hermite_tensor(n, q.cx[i], q.cy[i], cs, &H[off + i * mm]);
-    }
-    hipError_t e = hipMalloc(&m->dH, H.size() * sizeof(double));
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(m->dH, H.data(), H.size() * sizeof(double), hipMemcpyHostToDevice,
-                           g_stream);
-    if (e == hipSuccess) e = hipMalloc(&m->dOut, 8 * sizeof(double));
+    hipError_t e = hipMalloc(&m->dOut, 8 * sizeof(double));
     if (e == hipSuccess) e = hipStreamSynchronize(g_stream);
     if (e != hipSuccess) {
-        free_dev(reinterpret_cast<void *&>(m->dH));
+        free_dev(reinterpret_cast<void *&>(m->dOut));
         delete m;
         return fail(-3, "model allocation failed: %s", hipGetErrorString(e));
     }
@@ -612,7 +569,6 @@ int lbm_model_destroy(lbm_model m) {
     free_dev(reinterpret_cast<void *&>(m->dF));
     free_dev(reinterpret_cast<void *&>(m->dB));
     free_dev(reinterpret_cast<void *&>(m->dMw));
-    free_dev(reinterpret_cast<void *&>(m->dH));
     free_dev(reinterpret_cast<void *&>(m->dMei));
     free_dev(reinterpret_cast<void *&>(m->dPart));
     free_dev(reinterpret_cast<void *&>(m->dOut));

@@ -42,7 +42,6 @@ struct StepParams {
     int mrt_n;               // div(order, 2)
     T mrt_omo[5], mrt_om[5]; // 1 - 1/taus[n], 1/taus[n] for n = 1..4 (index n)
     T mrt_sn[5];             // s^n
-    const double *mrt_H;     // Hermite tables (see lbm_capi.cpp)
     T cs_inv;                // 1/s
     // iterative initialization (FP64 only): nonlinear equilibrium term (nx, ny, Q)
     const double *mei;
@@ -78,7 +77,6 @@ inline StepParams<float> to_float_params(const StepParams<double> &d) {
     f.dscale = static_cast<float>(d.dscale);
     f.dscale_inv = static_cast<float>(d.dscale_inv);
     f.mrt_n = 0;
-    f.mrt_H = nullptr;
     f.cs_inv = static_cast<float>(d.cs_inv);
     f.mei = nullptr;
     return f;

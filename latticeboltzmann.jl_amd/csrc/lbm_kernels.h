@@ -545,14 +545,129 @@ __device__ __forceinline__ void collide_node(T (&f)[QT<QID>::Q], const StepParam
 }
 
 // ---------------------------------------------------------------------------
-// regularised MRT (mrt.jl:56-118), FP64.  Hermite tables mrt_H: for order n
-// (2..4) a block of Q x 2^n values at offset Q*(0, 4, 12)[n-2], row i,
-// column k = column-major flattened tensor index.
+// regularised MRT (mrt.jl:56-118), FP64.
+//
+// The Hermite tensor H_n(c_i) (hermite_polynomials.jl:47-82) has entries
+// A - cs B (+ cs^2 C for n = 4) with cs = 1/s and integers A, B, C that only
+// depend on the abscissa and the flattened (column-major) tensor index k.
+// They are formed here at compile time, so every table value is either an
+// integer constant or one or two FP64 operations on cs: no table is read,
+// and products with an all-zero entry are left out.  Value by value this is
+// the host table's arithmetic (an integer term of zero adds an exact +0
+// there), and a left-out product only adds a signed zero to the sums, so
+// the populations equal the table-driven sums (mrt.jl:102, 106-115) bit for
+// bit, up to the sign of an exactly zero result.
 // ---------------------------------------------------------------------------
+struct HermInt {
+    int A, B, C;
+};
+LBM_HD constexpr int kdelta(int a, int b) { return a == b ? 1 : 0; }
+template <int QID>
+LBM_HD constexpr HermInt herm_int(int n, int i, int k) {
+    const int xi[2] = {QT<QID>::cx(i), QT<QID>::cy(i)};
+    const int a = k & 1, b = (k >> 1) & 1, g = (k >> 2) & 1, e = (k >> 3) & 1;
+    if (n == 2) return HermInt{xi[b] * xi[a], kdelta(a, b), 0};
+    if (n == 3)
+        return HermInt{xi[g] * xi[b] * xi[a],
+                       xi[a] * kdelta(b, g) + xi[b] * kdelta(a, g) + xi[g] * kdelta(a, b), 0};
+    return HermInt{xi[e] * xi[g] * xi[b] * xi[a],
+                   xi[a] * xi[b] * kdelta(g, e) + xi[a] * xi[g] * kdelta(b, e) +
+                       xi[a] * xi[e] * kdelta(b, g) + xi[b] * xi[g] * kdelta(a, e) +
+                       xi[b] * xi[e] * kdelta(a, g) + xi[g] * xi[e] * kdelta(a, b),
+                   kdelta(a, b) * kdelta(g, e) + kdelta(a, g) * kdelta(b, e) +
+                       kdelta(a, e) * kdelta(b, g)};
+}
+// the table value ((A - cs B) + cs^2 C), with the zero-integer terms dropped
+template <int A, int B, int C, class T>
+__device__ __forceinline__ T herm_value(T cs, T cs2) {
+    if constexpr (B == 0 && C == 0) return T(A);
+    else if constexpr (C == 0) return T(A) - cs * T(B);
+    else if constexpr (B == 0) return T(A) + cs2 * T(C);
+    else return (T(A) - cs * T(B)) + cs2 * T(C);
+}
+template <int QID, int N, int I, int K>
+LBM_HD constexpr bool herm_nonzero() {
+    constexpr HermInt h = herm_int<QID>(N, I, K);
+    return h.A != 0 || h.B != 0 || h.C != 0;
+}
+
+// a_f[k] = sum_i f_i H_n(c_i)[k] as a left fold over i (mrt.jl:102)
+template <int QID, int N, int K, class T, int... Is>
+__device__ __forceinline__ T mrt_project(const T (&f)[QT<QID>::Q], T cs, T cs2,
+                                         std::integer_sequence<int, Is...>) {
+    T a = T(0);
+    bool first = true;
+    auto term = [&](auto ic) {
+        constexpr int i = decltype(ic)::value;
+        if constexpr (herm_nonzero<QID, N, i, K>()) {
+            constexpr HermInt h = herm_int<QID>(N, i, K);
+            const T v = f[i] * herm_value<h.A, h.B, h.C>(cs, cs2);
+            a = first ? v : a + v;
+            first = false;
+        }
+    };
+    (term(std::integral_constant<int, Is>{}), ...);
+    return a;
+}
+
+// sum_k a_coll[k] H_n(c_i)[k] as a left fold over k (mrt.jl:106-115)
+template <int QID, int N, int I, class T, int... Ks>
+__device__ __forceinline__ T mrt_expand(const T (&ac)[16], T cs, T cs2,
+                                        std::integer_sequence<int, Ks...>) {
+    T d = T(0);
+    bool first = true;
+    auto term = [&](auto kc) {
+        constexpr int k = decltype(kc)::value;
+        if constexpr (herm_nonzero<QID, N, I, k>()) {
+            constexpr HermInt h = herm_int<QID>(N, I, k);
+            const T v = ac[k] * herm_value<h.A, h.B, h.C>(cs, cs2);
+            d = first ? v : d + v;
+            first = false;
+        }
+    };
+    (term(std::integral_constant<int, Ks>{}), ...);
+    return d;
+}
+
+template <int QID, int N, class T, int... Ks>
+__device__ __forceinline__ void mrt_coll_order(const T (&f)[QT<QID>::Q], const StepParams<T> &p,
+                                               T rho, const T (&u)[2], T cs, T cs2, T (&ac)[16],
+                                               std::integer_sequence<int, Ks...>) {
+    auto one = [&](auto kc) {
+        constexpr int k = decltype(kc)::value;
+        // a_eq (hermite.jl:45-77 at T = 1)
+        T pr = u[k & 1] * u[(k >> 1) & 1];
+        if constexpr (N >= 3) pr = pr * u[(k >> 2) & 1];
+        if constexpr (N >= 4) pr = pr * u[(k >> 3) & 1];
+        const T aeq = rho * pr;
+        const T a = mrt_project<QID, N, k>(f, cs, cs2,
+                                           std::make_integer_sequence<int, QT<QID>::Q>{});
+        ac[k] = p.mrt_omo[N] * a + p.mrt_om[N] * aeq;  // mrt.jl:104
+    };
+    (one(std::integral_constant<int, Ks>{}), ...);
+}
+
+// RN(x / 3) for finite x without a division: q = RN(x RN(1/3)) is within
+// one ulp of x / 3, its remainder x - 3q is exact in one fused operation, and
+// RN(q + rem RN(1/3)) is then the correctly rounded quotient (Markstein's
+// correction theorem; see Muller et al., Handbook of Floating-Point
+// Arithmetic, 2nd ed., sec. 4.9).  Dividing by 3! = 2 * 3 and 4! = 8 * 3
+// scales by a power of two first, which is exact away from the subnormals.
+__device__ __forceinline__ double div3_rn(double x) {
+    constexpr double r = 1.0 / 3.0;
+    const double q = x * r;
+    const double rem = __builtin_fma(-q, 3.0, x);
+    return __builtin_fma(rem, r, q);
+}
+
+template <class F, int... Is>
+__device__ __forceinline__ void mrt_each(F &&fn, std::integer_sequence<int, Is...>) {
+    (fn(std::integral_constant<int, Is>{}), ...);
+}
+
 template <int QID, class T>
 __device__ __forceinline__ void collide_mrt_node(T (&f)[QT<QID>::Q], const StepParams<T> &p,
                                                  int x, int y) {
-    constexpr int Q = QT<QID>::Q;
     constexpr int N = QT<QID>::ORDER / 2;
     T rho, ux, uy;
     node_moments<QID, T>(f, rho, ux, uy);
@@ -563,42 +678,33 @@ __device__ __forceinline__ void collide_mrt_node(T (&f)[QT<QID>::Q], const StepP
         uy = uy + p.tau_force * Fy;
     }
     const T u[2] = {ux, uy};
-    T acoll[3][16];
-#pragma unroll
-    for (int n = 2; n <= N; ++n) {
-        const int m = 1 << n;
-        const double *H = p.mrt_H + Q * (n == 2 ? 0 : (n == 3 ? 4 : 12));
-#pragma unroll
-        for (int k = 0; k < m; ++k) {
-            // a_eq (hermite.jl:45-77 at T = 1)
-            T pr = u[k & 1] * u[(k >> 1) & 1];
-            if (n >= 3) pr = pr * u[(k >> 2) & 1];
-            if (n >= 4) pr = pr * u[(k >> 3) & 1];
-            const T aeq = rho * pr;
-            // a_f = sum_i f_i H_n(c_i) (mrt.jl:102)
-            T a = f[0] * H[k] + f[1] * H[m + k];
-#pragma unroll
-            for (int i = 2; i < Q; ++i) a = a + f[i] * H[i * m + k];
-            acoll[n - 2][k] = p.mrt_omo[n] * a + p.mrt_om[n] * aeq;  // mrt.jl:104
-        }
-    }
-#pragma unroll
-    for (int i = 0; i < Q; ++i) {
+    const T cs = p.cs_inv, cs2 = cs * cs;
+    T ac2[16], ac3[16], ac4[16];
+    if constexpr (N >= 2)
+        mrt_coll_order<QID, 2>(f, p, rho, u, cs, cs2, ac2, std::make_integer_sequence<int, 4>{});
+    if constexpr (N >= 3)
+        mrt_coll_order<QID, 3>(f, p, rho, u, cs, cs2, ac3, std::make_integer_sequence<int, 8>{});
+    if constexpr (N >= 4)
+        mrt_coll_order<QID, 4>(f, p, rho, u, cs, cs2, ac4, std::make_integer_sequence<int, 16>{});
+    auto out = [&](auto ic) {
+        constexpr int i = decltype(ic)::value;
         const T ru = ux * static_cast<T>(QT<QID>::cx(i)) + uy * static_cast<T>(QT<QID>::cy(i));
         T hsum = T(0);
-#pragma unroll
-        for (int n = 2; n <= N; ++n) {
-            const int m = 1 << n;
-            const double *H = p.mrt_H + Q * (n == 2 ? 0 : (n == 3 ? 4 : 12));
-            T d = T(0);
-#pragma unroll
-            for (int k = 0; k < m; ++k) d = d + acoll[n - 2][k] * H[i * m + k];
-            const T fact = n == 2 ? T(2) : (n == 3 ? T(6) : T(24));
-            const T e = (p.mrt_sn[n] * d) / fact;
-            hsum = (n == 2) ? e : hsum + e;
+        if constexpr (N >= 2) {
+            const T d = mrt_expand<QID, 2, i>(ac2, cs, cs2, std::make_integer_sequence<int, 4>{});
+            hsum = (p.mrt_sn[2] * d) / T(2);
+        }
+        if constexpr (N >= 3) {
+            const T d = mrt_expand<QID, 3, i>(ac3, cs, cs2, std::make_integer_sequence<int, 8>{});
+            hsum = hsum + div3_rn(T(0.5) * (p.mrt_sn[3] * d));       // (s^3 d) / 3!
+        }
+        if constexpr (N >= 4) {
+            const T d = mrt_expand<QID, 4, i>(ac4, cs, cs2, std::make_integer_sequence<int, 16>{});
+            hsum = hsum + div3_rn(T(0.125) * (p.mrt_sn[4] * d));     // (s^4 d) / 4!
         }
         f[i] = p.w[i] * ((rho + (p.s * rho) * ru) + hsum);  // mrt.jl:106-115
-    }
+    };
+    mrt_each(out, std::make_integer_sequence<int, QT<QID>::Q>{});
 }
 
 // ---------------------------------------------------------------------------
