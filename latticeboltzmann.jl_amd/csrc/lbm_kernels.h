@@ -591,23 +591,32 @@ LBM_HD constexpr bool herm_nonzero() {
     return h.A != 0 || h.B != 0 || h.C != 0;
 }
 
-// a_f[k] = sum_i f_i H_n(c_i)[k] as a left fold over i (mrt.jl:102)
-template <int QID, int N, int K, class T, int... Is>
-__device__ __forceinline__ T mrt_project(const T (&f)[QT<QID>::Q], T cs, T cs2,
-                                         std::integer_sequence<int, Is...>) {
-    T a = T(0);
-    bool first = true;
-    auto term = [&](auto ic) {
-        constexpr int i = decltype(ic)::value;
-        if constexpr (herm_nonzero<QID, N, i, K>()) {
-            constexpr HermInt h = herm_int<QID>(N, i, K);
-            const T v = f[i] * herm_value<h.A, h.B, h.C>(cs, cs2);
-            a = first ? v : a + v;
-            first = false;
+// the first population with a nonzero entry k of H_n (Q if there is none)
+template <int QID, int N, int K>
+LBM_HD constexpr int herm_first() {
+    for (int i = 0; i < QT<QID>::Q; ++i) {
+        const HermInt h = herm_int<QID>(N, i, K);
+        if (h.A != 0 || h.B != 0 || h.C != 0) return i;
+    }
+    return QT<QID>::Q;
+}
+
+// population i's terms of a_f[k] = sum_i f_i H_n(c_i)[k] (mrt.jl:102): the
+// sums run over i in the outer loop, so each a_f[k] is the same left fold
+// over i while f_i is consumed as it goes
+template <int QID, int N, int I, class T, int... Ks>
+__device__ __forceinline__ void mrt_project_pop(const T fi, T cs, T cs2, T (&a)[16],
+                                                std::integer_sequence<int, Ks...>) {
+    auto term = [&](auto kc) {
+        constexpr int k = decltype(kc)::value;
+        if constexpr (herm_nonzero<QID, N, I, k>()) {
+            constexpr HermInt h = herm_int<QID>(N, I, k);
+            const T v = fi * herm_value<h.A, h.B, h.C>(cs, cs2);
+            if constexpr (herm_first<QID, N, k>() == I) a[k] = v;
+            else a[k] = a[k] + v;
         }
     };
-    (term(std::integral_constant<int, Is>{}), ...);
-    return a;
+    (term(std::integral_constant<int, Ks>{}), ...);
 }
 
 // sum_k a_coll[k] H_n(c_i)[k] as a left fold over k (mrt.jl:106-115)
@@ -629,10 +638,10 @@ __device__ __forceinline__ T mrt_expand(const T (&ac)[16], T cs, T cs2,
     return d;
 }
 
-template <int QID, int N, class T, int... Ks>
-__device__ __forceinline__ void mrt_coll_order(const T (&f)[QT<QID>::Q], const StepParams<T> &p,
-                                               T rho, const T (&u)[2], T cs, T cs2, T (&ac)[16],
-                                               std::integer_sequence<int, Ks...>) {
+// a_coll[k] = (1 - 1/tau_n) a_f[k] + (1/tau_n) a_eq[k] (mrt.jl:99-104)
+template <int N, class T, int... Ks>
+__device__ __forceinline__ void mrt_relax(const StepParams<T> &p, T rho, const T (&u)[2],
+                                          T (&ac)[16], std::integer_sequence<int, Ks...>) {
     auto one = [&](auto kc) {
         constexpr int k = decltype(kc)::value;
         // a_eq (hermite.jl:45-77 at T = 1)
@@ -640,9 +649,7 @@ __device__ __forceinline__ void mrt_coll_order(const T (&f)[QT<QID>::Q], const S
         if constexpr (N >= 3) pr = pr * u[(k >> 2) & 1];
         if constexpr (N >= 4) pr = pr * u[(k >> 3) & 1];
         const T aeq = rho * pr;
-        const T a = mrt_project<QID, N, k>(f, cs, cs2,
-                                           std::make_integer_sequence<int, QT<QID>::Q>{});
-        ac[k] = p.mrt_omo[N] * a + p.mrt_om[N] * aeq;  // mrt.jl:104
+        ac[k] = p.mrt_omo[N] * ac[k] + p.mrt_om[N] * aeq;  // mrt.jl:104
     };
     (one(std::integral_constant<int, Ks>{}), ...);
 }
@@ -680,12 +687,21 @@ __device__ __forceinline__ void collide_mrt_node(T (&f)[QT<QID>::Q], const StepP
     const T u[2] = {ux, uy};
     const T cs = p.cs_inv, cs2 = cs * cs;
     T ac2[16], ac3[16], ac4[16];
-    if constexpr (N >= 2)
-        mrt_coll_order<QID, 2>(f, p, rho, u, cs, cs2, ac2, std::make_integer_sequence<int, 4>{});
-    if constexpr (N >= 3)
-        mrt_coll_order<QID, 3>(f, p, rho, u, cs, cs2, ac3, std::make_integer_sequence<int, 8>{});
-    if constexpr (N >= 4)
-        mrt_coll_order<QID, 4>(f, p, rho, u, cs, cs2, ac4, std::make_integer_sequence<int, 16>{});
+#pragma unroll
+    for (int k = 0; k < 16; ++k) ac2[k] = ac3[k] = ac4[k] = T(0);
+    auto project = [&](auto ic) {
+        constexpr int i = decltype(ic)::value;
+        if constexpr (N >= 2)
+            mrt_project_pop<QID, 2, i>(f[i], cs, cs2, ac2, std::make_integer_sequence<int, 4>{});
+        if constexpr (N >= 3)
+            mrt_project_pop<QID, 3, i>(f[i], cs, cs2, ac3, std::make_integer_sequence<int, 8>{});
+        if constexpr (N >= 4)
+            mrt_project_pop<QID, 4, i>(f[i], cs, cs2, ac4, std::make_integer_sequence<int, 16>{});
+    };
+    mrt_each(project, std::make_integer_sequence<int, QT<QID>::Q>{});
+    if constexpr (N >= 2) mrt_relax<2>(p, rho, u, ac2, std::make_integer_sequence<int, 4>{});
+    if constexpr (N >= 3) mrt_relax<3>(p, rho, u, ac3, std::make_integer_sequence<int, 8>{});
+    if constexpr (N >= 4) mrt_relax<4>(p, rho, u, ac4, std::make_integer_sequence<int, 16>{});
     auto out = [&](auto ic) {
         constexpr int i = decltype(ic)::value;
         const T ru = ux * static_cast<T>(QT<QID>::cx(i)) + uy * static_cast<T>(QT<QID>::cy(i));
