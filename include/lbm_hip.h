@@ -179,6 +179,13 @@ int lbm_batch_download(lbm_batch b, int k, int role, double *f);
 /* threads per tile-block (64, 128, 256; 0 = 128) and grid size (0 = one block
  * per tile; smaller grids make the blocks stride over the tile table) */
 int lbm_batch_set_launch(lbm_batch b, int block, int grid);
+/* nodes per thread of the batch kernel (1, 2 or 4; default 1): a tile holds
+ * block x nodes consecutive nodes of one lattice, a lane nodes of them that
+ * lie block apart */
+int lbm_batch_set_nodes_per_thread(lbm_batch b, int nodes);
+/* the launch shape in use (after lbm_batch_autotune: the fastest one);
+ * any output may be NULL */
+int lbm_batch_get_launch(lbm_batch b, int *block, int *grid, int *nodes_per_thread);
 /* times launch shapes on the batch (state saved and restored), keeps the fastest */
 int lbm_batch_autotune(lbm_batch b, int trial_steps, int *block, int *grid, double *us_per_step);
 /* nsteps of every lattice; graph_chunk > 0 replays runs of graph_chunk fused

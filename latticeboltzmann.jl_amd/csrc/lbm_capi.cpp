@@ -1227,16 +1227,17 @@ struct lbm_batch_s {
     size_t pool_bytes = 0;
     int64_t nodes = 0;
     int parity_stream = 0;  // buffer holding f_stream
-    int block = 128, grid = 0;
+    int block = 128, grid = 0, vec = 1;  // threads per block, grid, nodes per thread
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
-    int g_chunk = 0, g_parity = -1, g_block = 0, g_grid = 0;
+    int g_chunk = 0, g_parity = -1, g_block = 0, g_grid = 0, g_vec = 0;
 };
 
 namespace {
 
 int batch_elem(const lbm_batch_s *b) { return storage_bytes(b->storage); }
 
+// the tile table for tiles of `block` nodes
 int batch_tiles(lbm_batch b, int block) {
     if (b->tile_block == block && b->dTiles) return 0;
     std::vector<lbm::BatchTile> tiles;
@@ -1253,10 +1254,8 @@ int batch_tiles(lbm_batch b, int block) {
     return 0;
 }
 
-// default grid: one block per tile.  Measured on MI355X with the 512-lattice
-// batch (89.6M nodes, FP32): 128-thread tiles, one block each, 1.55 ms per
-// step; persistent grids of 1/4 to 4 resident waves of blocks were slower
-// (1.9-2.0 ms: every tile waits on its table reads in series).
+// default grid: one block per tile; lbm_batch_autotune also times persistent
+// grids of 1/4 to 4 resident waves of blocks, which stride over the table.
 int batch_grid(const lbm_batch_s *b, int block) {
     (void)block;
     if (b->grid > 0) return b->grid;
@@ -1282,16 +1281,16 @@ lbm::StepParams<double> batch_params(const lbm_batch_s *b) {
 }
 
 int batch_launch(lbm_batch b, int op, int parity) {
-    if (int rc = batch_tiles(b, b->block)) return rc;
+    if (int rc = batch_tiles(b, b->block * b->vec)) return rc;
     const int grid = batch_grid(b, b->block);
     const lbm::StepParams<double> p = batch_params(b);
     int rc;
     if (b->storage == LBM_STORE_F64)
         rc = lbm::run_batch_f64_q9(op, b->dLats, b->dTiles, b->ntiles, b->pool, parity, b->block,
-                                   grid, p, g_stream);
+                                   grid, b->vec, p, g_stream);
     else
         rc = lbm::run_batch_f32_q9(op, b->dLats, b->dTiles, b->ntiles, b->pool, parity, b->block,
-                                   grid, lbm::to_float_params(p), g_stream);
+                                   grid, b->vec, lbm::to_float_params(p), g_stream);
     if (rc) return fail(-4, "batch kernel launch failed (%d)", rc);
     return 0;
 }
@@ -1309,10 +1308,10 @@ void batch_drop_graph(lbm_batch b) {
 int batch_graph(lbm_batch b, int chunk, int parity) {
     const int grid = batch_grid(b, b->block);
     if (b->gexec && b->g_chunk == chunk && b->g_parity == parity && b->g_block == b->block &&
-        b->g_grid == grid)
+        b->g_grid == grid && b->g_vec == b->vec)
         return 0;
     batch_drop_graph(b);
-    if (int rc = batch_tiles(b, b->block)) return rc;
+    if (int rc = batch_tiles(b, b->block * b->vec)) return rc;
     HIP_TRY(hipStreamBeginCapture(g_stream, hipStreamCaptureModeThreadLocal));
     int rc = 0, cur = parity;
     for (int k = 0; k < chunk && !rc; ++k) {
@@ -1336,6 +1335,7 @@ int batch_graph(lbm_batch b, int chunk, int parity) {
     b->g_parity = parity;
     b->g_block = b->block;
     b->g_grid = grid;
+    b->g_vec = b->vec;
     return 0;
 }
 
@@ -1362,7 +1362,8 @@ int lbm_batch_create(lbm_batch *out, int quadrature, int storage, int n, const i
     long long off = 0;
     for (int k = 0; k < n; ++k) {
         // node byte offsets within a lattice plane are 32-bit in the kernel
-        if (nx[k] < 1 || ny[k] < 1 || (long long)nx[k] * ny[k] > (1LL << 28)) {
+        if (nx[k] < 1 || ny[k] < 1 || nx[k] > (1 << 27) ||
+            (long long)(nx[k] + 31) * ny[k] > (1LL << 28)) {
             delete b;
             return fail(-1, "lattice %d: size %dx%d not supported", k, nx[k], ny[k]);
         }
@@ -1374,7 +1375,9 @@ int lbm_batch_create(lbm_batch *out, int quadrature, int storage, int n, const i
         lbm::BatchLattice L{};
         L.nx = nx[k];
         L.ny = ny[k];
-        L.plane = (long long)nx[k] * ny[k];
+        const int line = 128 / eb;  // elements per 128-byte line
+        L.pitch = (nx[k] + line - 1) / line * line;
+        L.plane = (long long)L.pitch * ny[k];
         L.off[0] = off;
         off += L.plane * b->q.Q;
         L.off[1] = off;
@@ -1383,12 +1386,12 @@ int lbm_batch_create(lbm_batch *out, int quadrature, int storage, int n, const i
         L.om = 1.0 / ts;
         L.oms = 1.0 / ts;
         L.oma = 1.0 / ta;
-        L.inv_nx = 1.0 / (double)nx[k];
+        L.inv_pitch = 1.0 / (double)L.pitch;
         L.coll = tau_a ? 1 : 0;
         b->lats.push_back(L);
         b->nx.push_back(nx[k]);
         b->ny.push_back(ny[k]);
-        b->nodes += L.plane;
+        b->nodes += (long long)nx[k] * ny[k];
     }
     b->pool_bytes = (size_t)off * eb;
     hipError_t e = hipMalloc(&b->pool, b->pool_bytes);
@@ -1432,26 +1435,37 @@ static int batch_xfer(lbm_batch b, int k, int buf, double *host, bool up) {
     if (k < 0 || k >= b->n) return fail(-1, "lattice index %d outside 0..%d", k, b->n - 1);
     if (!host) return fail(-1, "null host array");
     const lbm::BatchLattice &L = b->lats[k];
+    // the host array is dense (nx, ny, Q); the pool's rows are `pitch` long:
+    // ny * Q rows of nx values either way
     const size_t nb64 = (size_t)L.plane * b->q.Q * sizeof(double);
+    const size_t hrow = (size_t)L.nx * sizeof(double), drow = (size_t)L.pitch * sizeof(double);
+    const size_t rows = (size_t)L.ny * b->q.Q;
     const int eb = batch_elem(b);
     char *dst = static_cast<char *>(b->pool) + (size_t)L.off[buf] * eb;
     if (b->storage == LBM_STORE_F64) {
-        HIP_TRY(hipMemcpyAsync(up ? (void *)dst : (void *)host, up ? (const void *)host : dst, nb64,
-                               up ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost, g_stream));
+        if (up)
+            HIP_TRY(hipMemcpy2DAsync(dst, drow, host, hrow, hrow, rows, hipMemcpyHostToDevice,
+                                     g_stream));
+        else
+            HIP_TRY(hipMemcpy2DAsync(host, hrow, dst, drow, hrow, rows, hipMemcpyDeviceToHost,
+                                     g_stream));
         HIP_TRY(hipStreamSynchronize(g_stream));
         return 0;
     }
+    // FP32: through an FP64 buffer of the pool's shape (its padding slots are
+    // converted along and never read back)
     double *tmp = nullptr;
     HIP_TRY(hipMalloc(&tmp, nb64));
     lbm::StepParams<double> p = batch_params(b);
-    p.nx = L.nx;
+    p.nx = L.pitch;
     p.ny = L.ny;
     p.plane = L.plane;
     hipError_t e = hipSuccess;
-    if (up) e = hipMemcpyAsync(tmp, host, nb64, hipMemcpyHostToDevice, g_stream);
+    if (up) e = hipMemcpy2DAsync(tmp, drow, host, hrow, hrow, rows, hipMemcpyHostToDevice, g_stream);
     int rc = 0;
     if (e == hipSuccess) rc = lbm::dispatch_upload(b->qid, b->storage, tmp, dst, p, up ? 1 : 0, g_stream);
-    if (e == hipSuccess && !rc && !up) e = hipMemcpyAsync(host, tmp, nb64, hipMemcpyDeviceToHost, g_stream);
+    if (e == hipSuccess && !rc && !up)
+        e = hipMemcpy2DAsync(host, hrow, tmp, drow, hrow, rows, hipMemcpyDeviceToHost, g_stream);
     if (e == hipSuccess) e = hipStreamSynchronize(g_stream);
     (void)hipFree(tmp);
     if (e != hipSuccess) return fail(-3, "batch transfer failed: %s", hipGetErrorString(e));
@@ -1479,6 +1493,22 @@ int lbm_batch_set_launch(lbm_batch b, int block, int grid) {
     if (grid < 0) return fail(-1, "grid must be >= 0");
     b->block = block ? block : 128;
     b->grid = grid;
+    return 0;
+}
+
+int lbm_batch_set_nodes_per_thread(lbm_batch b, int nodes) {
+    if (!b) return fail(-1, "null batch");
+    if (nodes != 1 && nodes != 2 && nodes != 4)
+        return fail(-1, "nodes per thread must be 1, 2 or 4");
+    b->vec = nodes;
+    return 0;
+}
+
+int lbm_batch_get_launch(lbm_batch b, int *block, int *grid, int *nodes_per_thread) {
+    if (!b) return fail(-1, "null batch");
+    if (block) *block = b->block;
+    if (grid) *grid = b->grid;
+    if (nodes_per_thread) *nodes_per_thread = b->vec;
     return 0;
 }
 
@@ -1530,35 +1560,39 @@ int lbm_batch_autotune(lbm_batch b, int trial_steps, int *block, int *grid, doub
     void *save = nullptr;
     HIP_TRY(hipMalloc(&save, b->pool_bytes));
     int rc = lbm_field_copy_raw(save, b->pool, b->pool_bytes);
-    const int saved_parity = b->parity_stream, sb = b->block, sg = b->grid;
+    const int saved_parity = b->parity_stream, sb = b->block, sg = b->grid, sv = b->vec;
     double best = 1e300;
-    int bb = 256, bg = 0;
+    int bb = 256, bg = 0, bv = 1;
     int dev = 0, cus = 256;
     if (hipGetDevice(&dev) == hipSuccess)
         (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     const int blocks[3] = {64, 128, 256};
-    for (int bi = 0; !rc && bi < 3; ++bi) {
-        const int per_cu = 2048 / blocks[bi];
-        if (int r2 = batch_tiles(b, blocks[bi])) {
-            rc = r2;
-            break;
-        }
-        // persistent grids of 1/4 .. 4 resident waves of blocks, and one block per tile
-        const int grids[5] = {cus * per_cu / 4, cus * per_cu / 2, cus * per_cu, cus * per_cu * 4,
-                              b->ntiles};
-        for (int gi = 0; !rc && gi < 5; ++gi) {
-            b->block = blocks[bi];
-            b->grid = grids[gi];
-            double ms = 0.0;
-            rc = lbm_batch_simulate(b, trial_steps + 1, 0, &ms);
-            const double us = 1e3 * ms / trial_steps;
-            if (!rc && us < best) {
-                best = us;
-                bb = blocks[bi];
-                bg = grids[gi];
+    const int vecs[3] = {1, 2, 4};
+    for (int vi = 0; !rc && vi < 3; ++vi)
+        for (int bi = 0; !rc && bi < 3; ++bi) {
+            const int per_cu = 2048 / blocks[bi];
+            if (int r2 = batch_tiles(b, blocks[bi] * vecs[vi])) {
+                rc = r2;
+                break;
+            }
+            // persistent grids of 1/4 .. 4 resident waves of blocks, and one block per tile
+            const int grids[5] = {cus * per_cu / 4, cus * per_cu / 2, cus * per_cu,
+                                  cus * per_cu * 4, b->ntiles};
+            for (int gi = 0; !rc && gi < 5; ++gi) {
+                b->block = blocks[bi];
+                b->grid = grids[gi];
+                b->vec = vecs[vi];
+                double ms = 0.0;
+                rc = lbm_batch_simulate(b, trial_steps + 1, 0, &ms);
+                const double us = 1e3 * ms / trial_steps;
+                if (!rc && us < best) {
+                    best = us;
+                    bb = blocks[bi];
+                    bg = grids[gi];
+                    bv = vecs[vi];
+                }
             }
         }
-    }
     if (!rc) rc = lbm_field_copy_raw(b->pool, save, b->pool_bytes);
     (void)hipStreamSynchronize(g_stream);
     (void)hipFree(save);
@@ -1567,10 +1601,12 @@ int lbm_batch_autotune(lbm_batch b, int trial_steps, int *block, int *grid, doub
     if (rc) {
         b->block = sb;
         b->grid = sg;
+        b->vec = sv;
         return rc;
     }
     b->block = bb;
     b->grid = bg;
+    b->vec = bv;
     if (block) *block = bb;
     if (grid) *grid = bg;
     if (us_per_step) *us_per_step = best;

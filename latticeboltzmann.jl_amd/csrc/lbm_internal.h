@@ -83,15 +83,18 @@ inline StepParams<float> to_float_params(const StepParams<double> &d) {
 }
 
 // Batched lattices: one entry per lattice and one per tile of consecutive
-// nodes (row-major within the lattice).  Offsets are in elements of the pool.
+// node slots (row-major within the lattice).  Rows are padded to `pitch`
+// slots, a multiple of 128 bytes, so every row and plane starts on a cache
+// line; the slots x = nx .. pitch - 1 of a row hold nothing.  Offsets are in
+// elements of the pool.
 struct BatchLattice {
     int nx, ny;
-    long long plane;
-    long long off[2];  // ping-pong buffers
+    long long plane;           // pitch * ny: the stride between populations
+    long long off[2];          // ping-pong buffers
     double omo, om, oms, oma;  // SRT / TRT relaxation constants
-    double inv_nx;             // 1 / nx (row of a node index, corrected exactly)
+    double inv_pitch;          // 1 / pitch (row of a slot index, corrected exactly)
     int coll;                  // 0 SRT, 1 TRT
-    int pad;
+    int pitch;                 // row stride in elements
 };
 
 struct BatchTile {
@@ -145,12 +148,13 @@ int dispatch_lp(const Launch &L, const StepParams<double> &p, hipStream_t s);
     LBM_DECLARE_RUN(TAG, T, 13) LBM_DECLARE_RUN(TAG, T, 17) LBM_DECLARE_RUN(TAG, T, 21) \
     LBM_DECLARE_RUN(TAG, T, 37)
 // batched periodic D2Q9 step: op 0 collide, 1 fused step, 2 stream
+// (vec nodes per thread: 1, 2 or 4; tiles of vec * block nodes)
 int run_batch_f64_q9(int op, const BatchLattice *lats, const BatchTile *tiles, int ntiles,
-                     void *pool, int parity, int block, int grid, const StepParams<double> &p,
-                     hipStream_t s);
+                     void *pool, int parity, int block, int grid, int vec,
+                     const StepParams<double> &p, hipStream_t s);
 int run_batch_f32_q9(int op, const BatchLattice *lats, const BatchTile *tiles, int ntiles,
-                     void *pool, int parity, int block, int grid, const StepParams<float> &p,
-                     hipStream_t s);
+                     void *pool, int parity, int block, int grid, int vec,
+                     const StepParams<float> &p, hipStream_t s);
 
 LBM_DECLARE_ALL_Q(f64, double)
 LBM_DECLARE_ALL_Q(f32, float)

@@ -1571,13 +1571,15 @@ int launch_op(const Launch &L, const StepParams<typename ST::T> &p, hipStream_t 
 
 // ---------------------------------------------------------------------------
 // batched independent periodic lattices: a persistent grid strides over a
-// table of tiles (blockDim.x consecutive nodes of one lattice each); lattice
-// geometry and relaxation times come from the lattice table (uniform per
-// tile, so they live in scalar registers and the plane bases are scalar: a
-// lane adds a 32-bit offset).  OP: 0 collide, 1 fused step, 2 stream.  All
+// table of tiles (VEC * blockDim.x consecutive nodes of one lattice each);
+// lattice geometry and relaxation times come from the lattice table (uniform
+// per tile, so they live in scalar registers and the plane bases are scalar:
+// a lane adds a 32-bit offset).  A lane holds VEC nodes blockDim.x apart, so
+// every load instruction stays coalesced, and issues all their loads before
+// it collides any of them.  OP: 0 collide, 1 fused step, 2 stream.  All
 // lattices share the ping-pong parity; stores are non-temporal.
 // ---------------------------------------------------------------------------
-template <int QID, class ST, int OP>
+template <int QID, class ST, int OP, int VEC>
 __global__ void __launch_bounds__(256)
 k_batch(const BatchLattice *__restrict__ lats, const BatchTile *__restrict__ tiles, int ntiles,
         typename ST::store_t *__restrict__ pool, int parity,
@@ -1585,10 +1587,13 @@ k_batch(const BatchLattice *__restrict__ lats, const BatchTile *__restrict__ til
     using T = typename ST::T;
     using S = typename ST::store_t;
     constexpr int Q = QT<QID>::Q;
-    constexpr int M = QT<QID>::MAXC;
+    static_assert(QT<QID>::MAXC == 1, "batches pull from the nearest neighbours only");
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const BatchTile tile = tiles[t];
-        const BatchLattice L = lats[tile.lat];
+        // the lattice record is read in place, field by field (a local copy
+        // indexed by the parity would live in memory): with a wave-uniform
+        // index every field is a scalar load and the plane bases stay scalar
+        const BatchLattice &L = lats[__builtin_amdgcn_readfirstlane(tile.lat)];
         StepParams<T> p = base;
         p.nx = L.nx;
         p.ny = L.ny;
@@ -1599,75 +1604,98 @@ k_batch(const BatchLattice *__restrict__ lats, const BatchTile *__restrict__ til
         p.oma = static_cast<T>(L.oma);
         const S *src = pool + L.off[parity];
         S *dst = pool + L.off[parity ^ 1];
-        const int n = tile.n0 + (int)threadIdx.x;
-        if (n >= (int)L.plane) continue;
-        int y = (int)((double)n * L.inv_nx);
-        int x = n - y * L.nx;
-        if (x < 0) {
-            --y;
-            x += L.nx;
-        } else if (x >= L.nx) {
-            ++y;
-            x -= L.nx;
-        }
-        T f[Q];
-        if constexpr (OP == 0) {
-            auto ld = [&](auto Ic) {
-                constexpr int i = decltype(Ic)::value;
-                f[i] = ST::decode(*at32(src + p.plane * i, n), i, p);
-            };
-            adj_each_node<QID>(ld);
-        } else if constexpr (M == 1) {
-            // periodic pull without a branch: the wrapped neighbour columns and
-            // rows are selected per node, so the lanes of a wave that holds a
-            // lattice edge (most waves, for lattices narrower than a few
-            // hundred nodes) do not split into an interior and an edge path
-            const int xm = x == 0 ? L.nx - 1 : x - 1, xp = x == L.nx - 1 ? 0 : x + 1;
-            const int ym = y == 0 ? L.ny - 1 : y - 1, yp = y == L.ny - 1 ? 0 : y + 1;
-            auto ld = [&](auto Ic) {
-                constexpr int i = decltype(Ic)::value;
-                constexpr int cx = QT<QID>::cx(i), cy = QT<QID>::cy(i);
-                const int sx = cx == 0 ? x : (cx > 0 ? xm : xp);
-                const int sy = cy == 0 ? y : (cy > 0 ? ym : yp);
-                f[i] = ST::decode(*at32(src + p.plane * i, sx + L.nx * sy), i, p);
-            };
-            adj_each_node<QID>(ld);
-        } else {
-            if (x >= M && x < L.nx - M && y >= M && y < L.ny - M) {
+        const int last = (int)L.plane - 1, pitch = L.pitch;
+        T f[VEC][Q];
+        int nn[VEC], xs[VEC], ys[VEC];
+        bool live[VEC];
+#pragma unroll
+        for (int v = 0; v < VEC; ++v) {
+            // a lane past the lattice's end or on a row's padding loads the
+            // row's last node and stores nothing
+            const int n0 = tile.n0 + (int)threadIdx.x + v * (int)blockDim.x;
+            const int s0 = n0 <= last ? n0 : last;
+            int y = (int)((double)s0 * L.inv_pitch);
+            int x = s0 - y * pitch;
+            if (x < 0) {
+                --y;
+                x += pitch;
+            } else if (x >= pitch) {
+                ++y;
+                x -= pitch;
+            }
+            live[v] = n0 <= last && x < L.nx;
+            if (x >= L.nx) x = L.nx - 1;
+            const int n = x + pitch * y;
+            nn[v] = n;
+            xs[v] = x;
+            ys[v] = y;
+            if constexpr (OP == 0) {
                 auto ld = [&](auto Ic) {
                     constexpr int i = decltype(Ic)::value;
-                    constexpr int cx = QT<QID>::cx(i), cy = QT<QID>::cy(i);
-                    f[i] = ST::decode(*at32(src + p.plane * i, n - cx - L.nx * cy), i, p);
+                    f[v][i] = ST::decode(*at32(src + p.plane * i, n), i, p);
                 };
                 adj_each_node<QID>(ld);
             } else {
-                pull_seq<QID, ST>(src, p, x, y, f, LBM_SEQ);
+                // periodic pull without a branch: the wrapped neighbour columns
+                // and rows are selected per node, so the lanes of a wave that
+                // holds a lattice edge (most waves, for lattices narrower than
+                // a few hundred nodes) do not split into an interior and an
+                // edge path
+                const int xm = x == 0 ? L.nx - 1 : x - 1, xp = x == L.nx - 1 ? 0 : x + 1;
+                const int ym = y == 0 ? L.ny - 1 : y - 1, yp = y == L.ny - 1 ? 0 : y + 1;
+                auto ld = [&](auto Ic) {
+                    constexpr int i = decltype(Ic)::value;
+                    constexpr int cx = QT<QID>::cx(i), cy = QT<QID>::cy(i);
+                    const int sx = cx == 0 ? x : (cx > 0 ? xm : xp);
+                    const int sy = cy == 0 ? y : (cy > 0 ? ym : yp);
+                    f[v][i] = ST::decode(*at32(src + p.plane * i, sx + pitch * sy), i, p);
+                };
+                adj_each_node<QID>(ld);
             }
         }
-        if constexpr (OP != 2) {
-            if (L.coll == 0) collide_any<QID, 0>(f, p, x, y);
-            else collide_any<QID, 1>(f, p, x, y);
+#pragma unroll
+        for (int v = 0; v < VEC; ++v) {
+            if constexpr (OP != 2) {
+                if (L.coll == 0) collide_any<QID, 0>(f[v], p, xs[v], ys[v]);
+                else collide_any<QID, 1>(f[v], p, xs[v], ys[v]);
+            }
+            if (live[v]) {
+                auto st = [&](auto Ic) {
+                    constexpr int i = decltype(Ic)::value;
+                    Pack<S, 1> pk;
+                    pk.v[0] = ST::encode(f[v][i], i, p);
+                    put_pack<true>(at32(dst + p.plane * i, nn[v]), pk);
+                };
+                adj_each_node<QID>(st);
+            }
         }
-        auto st = [&](auto Ic) {
-            constexpr int i = decltype(Ic)::value;
-            Pack<S, 1> pk;
-            pk.v[0] = ST::encode(f[i], i, p);
-            put_pack<true>(at32(dst + p.plane * i, n), pk);
-        };
-        adj_each_node<QID>(st);
     }
+}
+
+template <int QID, class ST, int VEC>
+int launch_batch(int op, const BatchLattice *lats, const BatchTile *tiles, int ntiles,
+                 typename ST::store_t *pl, int parity, int block, int grid,
+                 const StepParams<typename ST::T> &p, hipStream_t s) {
+    if (op == 0) k_batch<QID, ST, 0, VEC><<<grid, block, 0, s>>>(lats, tiles, ntiles, pl, parity, p);
+    else if (op == 1) k_batch<QID, ST, 1, VEC><<<grid, block, 0, s>>>(lats, tiles, ntiles, pl, parity, p);
+    else k_batch<QID, ST, 2, VEC><<<grid, block, 0, s>>>(lats, tiles, ntiles, pl, parity, p);
+    return hipGetLastError() == hipSuccess ? 0 : -4;
 }
 
 #define LBM_DEFINE_BATCH(TAG, ST, QID)                                                       \
     int run_batch_##TAG##_q##QID(int op, const BatchLattice *lats, const BatchTile *tiles,   \
                                  int ntiles, void *pool, int parity, int block, int grid,    \
-                                 const StepParams<typename ST::T> &p, hipStream_t s) {      \
+                                 int vec, const StepParams<typename ST::T> &p,               \
+                                 hipStream_t s) {                                            \
         using S = typename ST::store_t;                                                      \
         S *pl = static_cast<S *>(pool);                                                      \
-        if (op == 0) k_batch<QID, ST, 0><<<grid, block, 0, s>>>(lats, tiles, ntiles, pl, parity, p); \
-        else if (op == 1) k_batch<QID, ST, 1><<<grid, block, 0, s>>>(lats, tiles, ntiles, pl, parity, p); \
-        else k_batch<QID, ST, 2><<<grid, block, 0, s>>>(lats, tiles, ntiles, pl, parity, p); \
-        return hipGetLastError() == hipSuccess ? 0 : -4;                                     \
+        if (vec == 1)                                                                        \
+            return launch_batch<QID, ST, 1>(op, lats, tiles, ntiles, pl, parity, block, grid, p, s); \
+        if (vec == 2)                                                                        \
+            return launch_batch<QID, ST, 2>(op, lats, tiles, ntiles, pl, parity, block, grid, p, s); \
+        if (vec == 4)                                                                        \
+            return launch_batch<QID, ST, 4>(op, lats, tiles, ntiles, pl, parity, block, grid, p, s); \
+        return -2;                                                                           \
     }
 
 // One translation unit per (storage, quadrature group) instantiates these

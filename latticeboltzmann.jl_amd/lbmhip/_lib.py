@@ -32,7 +32,8 @@ EXPORTS = [
     "lbm_collide", "lbm_stream", "lbm_apply_bcs", "lbm_stream_bc", "lbm_simulate",
     "lbm_simulate_timed", "lbm_moments", "lbm_reduce",
     "lbm_batch_create", "lbm_batch_destroy", "lbm_batch_info", "lbm_batch_upload",
-    "lbm_batch_download", "lbm_batch_set_launch", "lbm_batch_autotune", "lbm_batch_simulate",
+    "lbm_batch_download", "lbm_batch_set_launch", "lbm_batch_set_nodes_per_thread",
+    "lbm_batch_get_launch", "lbm_batch_autotune", "lbm_batch_simulate",
     "lbm_adjoint_create", "lbm_adjoint_destroy", "lbm_adjoint_info", "lbm_adjoint_loss",
     "lbm_adjoint_gradient",
     "lbm_expected_create", "lbm_expected_destroy", "lbm_expected_upload", "lbm_track_errors",
@@ -119,6 +120,8 @@ def lib() -> ctypes.CDLL:
     L.lbm_batch_upload.argtypes = [vp, i, dp]
     L.lbm_batch_download.argtypes = [vp, i, i, dp]
     L.lbm_batch_set_launch.argtypes = [vp, i, i]
+    L.lbm_batch_set_nodes_per_thread.argtypes = [vp, i]
+    L.lbm_batch_get_launch.argtypes = [vp, ip, ip, ip]
     L.lbm_batch_autotune.argtypes = [vp, i, ip, ip, dp]
     L.lbm_batch_simulate.argtypes = [vp, i64, i, dp]
     L.lbm_adjoint_create.argtypes = [ctypes.POINTER(vp), vp, i, i, i, d, i64, i]

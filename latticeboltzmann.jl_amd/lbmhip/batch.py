@@ -54,9 +54,20 @@ class LatticeBatch:
                    "lbm_batch_download")
         return out
 
-    def set_launch(self, block: int = 0, grid: int = 0) -> None:
+    def set_launch(self, block: int = 0, grid: int = 0, nodes_per_thread: int = None) -> None:
         _lib.check(_lib.lib().lbm_batch_set_launch(self._h, int(block), int(grid)),
                    "lbm_batch_set_launch")
+        if nodes_per_thread is not None:
+            _lib.check(_lib.lib().lbm_batch_set_nodes_per_thread(self._h, int(nodes_per_thread)),
+                       "lbm_batch_set_nodes_per_thread")
+
+    def launch(self) -> dict:
+        """The launch shape in use: threads per block, grid (0 = one block per
+        tile) and nodes per thread."""
+        b, g, v = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _lib.check(_lib.lib().lbm_batch_get_launch(self._h, ctypes.byref(b), ctypes.byref(g),
+                                                   ctypes.byref(v)), "lbm_batch_get_launch")
+        return {"block": b.value, "grid": g.value, "nodes_per_thread": v.value}
 
     def autotune(self, trial_steps: int = 5) -> dict:
         b, g = ctypes.c_int(), ctypes.c_int()
@@ -64,7 +75,9 @@ class LatticeBatch:
         _lib.check(_lib.lib().lbm_batch_autotune(self._h, int(trial_steps), ctypes.byref(b),
                                                  ctypes.byref(g), ctypes.byref(us)),
                    "lbm_batch_autotune")
-        return {"block": b.value, "grid": g.value, "us_per_step": us.value}
+        out = self.launch()
+        out["us_per_step"] = us.value
+        return out
 
     def simulate(self, nsteps: int, graph_chunk: int = 100) -> float:
         """Advances every lattice nsteps; returns the milliseconds of the fused steps."""

@@ -35,6 +35,35 @@ def test_batch_f64_bit_exact(oracle_lib, chunk, trt):
         assert np.array_equal(b.download(k, "f_collision"), C), k
 
 
+@pytest.mark.parametrize("nodes", [1, 2, 4])
+@pytest.mark.parametrize("block,grid", [(64, 0), (256, 0), (128, 3)])
+def test_batch_launch_shapes_bit_exact(oracle_lib, nodes, block, grid):
+    """every tile shape (block x nodes per thread), one block per tile or a
+    3-block grid striding over the table, gives the oracle's bits; lattices
+    smaller than one tile leave lanes that load a clamped node and store
+    nothing"""
+    b, f0, ts, ta = _batch("f64")
+    b.set_launch(block, grid, nodes_per_thread=nodes)
+    assert b.launch() == {"block": block, "grid": grid, "nodes_per_thread": nodes}
+    b.simulate(7, graph_chunk=0)
+    b.simulate(6, graph_chunk=3)
+    for k, f in enumerate(f0):
+        S, C = f.copy(order="F"), f.copy(order="F")
+        oracle_lib.simulate(9, S, C, 1, [ts[k], ta[k]], 13)
+        assert np.array_equal(b.download(k, "f_stream"), S), k
+        assert np.array_equal(b.download(k, "f_collision"), C), k
+
+
+def test_batch_launch_errors():
+    import lbmhip as L
+    b, _, _, _ = _batch("f32")
+    with pytest.raises(L.LbmError):
+        b.set_launch(96, 0)
+    with pytest.raises(L.LbmError):
+        b.set_launch(128, 0, nodes_per_thread=3)
+    assert b.launch()["nodes_per_thread"] == 1
+
+
 def test_batch_continues_across_calls(oracle_lib):
     b, f0, ts, ta = _batch("f64")
     b.simulate(5, graph_chunk=2)
@@ -70,6 +99,7 @@ def test_batch_autotune_keeps_state(oracle_lib):
     b, f0, ts, ta = _batch("f64")
     best = b.autotune(2)
     assert best["block"] in (64, 128, 256) and best["grid"] > 0
+    assert best["nodes_per_thread"] in (1, 2, 4)
     for k, f in enumerate(f0):
         assert np.array_equal(b.download(k), f), k
     b.simulate(3)
