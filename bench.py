@@ -330,7 +330,8 @@ def precision_leg(L, steps, nx=8192, ny=2048, tau=0.8, qids=(9, 17, 37),
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2000)
+    # the Taylor-Green workload: 5000 timed steps after 50 warm-up steps
+    ap.add_argument("--steps", type=int, default=5000)
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--n", type=int, default=4096)
     ap.add_argument("--tau", type=float, default=0.6)
