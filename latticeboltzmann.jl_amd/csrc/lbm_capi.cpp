@@ -315,8 +315,14 @@ void auto_launch(const lbm_model_s *m, const lbm_field_s *f, int &block, int &ro
     // 16-bit deviations on the 21- and 37-velocity stencils: 64-thread blocks
     // (D2Q37 FP16-DEV 8192 x 2048: 508 us against 534 us with 256)
     const bool wide16 = storage_bytes(f->storage) == 2 && f->Q > 17 && !rows_mode(f->storage);
-    block = m->launch_block > 0 ? m->launch_block : (f64 ? 128 : (wide16 ? 64 : 256));
-    rows = m->launch_rows > 0 ? m->launch_rows : (f64 ? 2 : 1);
+    // MRT, non-temporal stores: 128 x 2 on D2Q4/5/9/13 (D2Q9 4096^2: 423 us),
+    // 256 x 1 on D2Q17 (8192 x 2048: 816 us), 64 x 1 on D2Q21/37, whose
+    // kernel holds 229 VGPRs (D2Q37 8192 x 2048: 2.19 ms against 2.42 with 256)
+    const bool mrt = m->coll == LBM_MRT;
+    const int mrt_block = f->Q > 17 ? 64 : (f->Q > 13 ? 256 : 128);
+    block = m->launch_block > 0 ? m->launch_block
+                                : (mrt ? mrt_block : (f64 ? 128 : (wide16 ? 64 : 256)));
+    rows = m->launch_rows > 0 ? m->launch_rows : ((f64 || (mrt && mrt_block == 128)) ? 2 : 1);
     if (block * rows > 256) rows = 256 / block;
     vec = m->launch_vec;
     nt = m->launch_nt;
@@ -340,7 +346,7 @@ void auto_launch(const lbm_model_s *m, const lbm_field_s *f, int &block, int &ro
                 }
         }
     }
-    if (nt < 0) nt = (m->coll == LBM_MRT) ? 0 : 1;
+    if (nt < 0) nt = 1;
 }
 
 int run(lbm::Op op, lbm_model m, lbm_field src, lbm_field dst, bool with_bcs) {

@@ -1549,8 +1549,14 @@ int launch_op(const Launch &L, const StepParams<typename ST::T> &p, hipStream_t 
     case Op::Fused:
         if (L.coll == 3) return -2;  // collide-only model: the host runs collide + stream
         if (L.coll == 2) {
-            if constexpr (ST::id == 0) k_fused<QID, 2, ST><<<grid, block, 0, s>>>(src, dst, p);
-            else return -2;
+            // MRT (FP64 only): with non-temporal stores through the row-shaped
+            // kernel, one node per thread
+            if constexpr (ST::id == 0) {
+                if (L.nt) return launch_fused_vn<QID, 2, ST, 1>(L, p, s);
+                k_fused<QID, 2, ST><<<grid, block, 0, s>>>(src, dst, p);
+            } else {
+                return -2;
+            }
             break;
         }
         if (L.vec > 1 || L.nt || ST::kNeq) return launch_fused_v<QID, ST>(L, p, s);

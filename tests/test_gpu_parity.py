@@ -299,6 +299,30 @@ def test_vectorized_fused_bit_exact(oracle_lib, qid, nt, bcname):
     _assert_same(model.f_collision.to_numpy(), C, f"vec2 nt={nt} D2Q{qid} {bcname} (C)")
 
 
+@pytest.mark.parametrize("qid", QIDS)
+@pytest.mark.parametrize("launch", [(64, 2, 1, 1), (128, 1, 1, 1), (64, 1, 1, 0)])
+@pytest.mark.parametrize("bcname", ["periodic", "cavity"])
+def test_mrt_launch_shapes_bit_exact(oracle_lib, qid, launch, bcname):
+    """MRT through the row-shaped kernel with non-temporal stores (nt = 1) and
+    through the plain one-node kernel (nt = 0), with a field force and BCs:
+    the oracle's bits either way."""
+    L = _L()
+    nx, ny = 21, 10
+    f0 = random_populations(qid, nx, ny, seed=17)
+    F = force_for("field", nx, ny)
+    cm, taus = _cm(2, qid, F)
+    bcs = bc_dicts(bcname, nx, ny)
+    q = _q(qid)
+    model = L.LatticeBoltzmannModel.from_populations(q, f0, cm, to_lbmhip(bcs))
+    model.launch = launch
+    L.simulate(model, range(6))
+    S = f0.copy(order="F")
+    C = f0.copy(order="F")
+    oracle_lib.simulate(qid, S, C, 2, taus, 6, force=F, bcs=bcs)
+    _assert_same(model.f_stream.to_numpy(), S, f"MRT {launch} D2Q{qid} {bcname}")
+    _assert_same(model.f_collision.to_numpy(), C, f"MRT {launch} D2Q{qid} {bcname} (C)")
+
+
 def test_vec_requires_divisible_nx():
     L = _L()
     q = L.D2Q9()
