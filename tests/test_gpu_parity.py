@@ -323,6 +323,30 @@ def test_mrt_launch_shapes_bit_exact(oracle_lib, qid, launch, bcname):
     _assert_same(model.f_collision.to_numpy(), C, f"MRT {launch} D2Q{qid} {bcname} (C)")
 
 
+@pytest.mark.parametrize("qid", QIDS)
+@pytest.mark.parametrize("coll", [0, 1, 2])
+@pytest.mark.parametrize("nx", [32, 33])
+def test_row_shaped_and_flat_kernels_bit_exact(oracle_lib, qid, coll, nx):
+    """One node per thread: rows of whole 128-byte lines (nx = 32 in FP64) run
+    the row-shaped kernel, other widths (nx = 33) the flat-index kernel; with
+    a field force and the cavity BC list both give the oracle's bits."""
+    L = _L()
+    ny = 9
+    f0 = random_populations(qid, nx, ny, seed=23 + coll)
+    F = force_for("field", nx, ny)
+    cm, taus = _cm(coll, qid, F)
+    bcs = bc_dicts("cavity", nx, ny)
+    q = _q(qid)
+    model = L.LatticeBoltzmannModel.from_populations(q, f0, cm, to_lbmhip(bcs))
+    model.launch = (64, 1, 1, 1)
+    L.simulate(model, range(5))
+    S = f0.copy(order="F")
+    C = f0.copy(order="F")
+    oracle_lib.simulate(qid, S, C, coll, taus, 5, force=F, bcs=bcs)
+    _assert_same(model.f_stream.to_numpy(), S, f"nx={nx} coll={coll} D2Q{qid}")
+    _assert_same(model.f_collision.to_numpy(), C, f"nx={nx} coll={coll} D2Q{qid} (C)")
+
+
 def test_vec_requires_divisible_nx():
     L = _L()
     q = L.D2Q9()
