@@ -4,7 +4,8 @@ A step is one time step of the lattice (collide -> stream -> BCs) over the
 whole 4096^2 lattice, i.e. one fused pull collide-stream kernel launch.
 ``value`` is the plain FP64 throughput in MLUPS (million lattice updates per
 second) over K timed steps, populations resident in HBM.  The same run with
-FP16-deviation storage (FP32 arithmetic) is reported beside it with its
+each reduced storage (FP16/BF16 non-equilibrium records, FP16/BF16
+deviations, shifted FP32; FP32 arithmetic) is reported beside it with its
 relative L2 velocity error against the FP64 run.
 
 ``roofline`` prices the fused kernel against MI355X's 8 TB/s: algorithmic
@@ -413,10 +414,23 @@ def main():
     }
 
     if not a.no_fp16:
-        # 16-bit storages: rest-state deviations (f16dev) and non-equilibrium
-        # parts about a binary16 equilibrium record (f16neq: 2 Q + 8 bytes per node)
+        # the reduced storages: non-equilibrium parts about a 16-bit equilibrium
+        # record (*neq: 2 Q + 8 bytes per node), rest-state deviations (*dev),
+        # and shifted FP32
+        kernels = {
+            "f16neq": "k_fused_neq_rows<9,TRT,F16Neq> (8 rows/thread, LDS record window, "
+                      "edge nodes in its last block column)",
+            "bf16neq": "k_fused_neq_rows<9,TRT,BF16Neq> (8 rows/thread, LDS record window, "
+                       "edge nodes in its last block column)",
+            "f16dev": "k_fused_v<9,TRT,F16Dev> (4 nodes/thread)",
+            "bf16dev": "k_fused_v<9,TRT,BF16Dev> (4 nodes/thread)",
+            "f32": "k_fused_v<9,TRT,F32> (256x1 blocks, 1 node/thread)",
+        }
         for storage, key, per_node in (("f16neq", "fp16neq", 2 * q.Q + 8),
-                                       ("f16dev", "fp16dev", 2 * q.Q)):
+                                       ("f16dev", "fp16dev", 2 * q.Q),
+                                       ("bf16neq", "bf16neq", 2 * q.Q + 8),
+                                       ("bf16dev", "bf16dev", 2 * q.Q),
+                                       ("f32", "fp32", 4 * q.Q)):
             print(f"[bench] {storage}: {W} warmup + {K} timed steps", file=sys.stderr,
                   flush=True)
             m16, wall16, kms16, nl16 = run_storage(L, q, f0, cm, storage, K, W)
@@ -438,10 +452,7 @@ def main():
                 "roofline": {"bound": "hbm", "achieved": round(gbs16, 1), "peak": HBM_PEAK_GBS,
                              "unit": "GB/s", "frac": round(gbs16 / HBM_PEAK_GBS, 4),
                              "traffic": traffic_for(f"d2q9_trt_{storage}_4096"),
-                             "kernel": ("k_fused_neq_rows<9,TRT,F16Neq> (8 rows/thread, LDS "
-                                        "record window, edge nodes in its last block column)"
-                                        if storage == "f16neq"
-                                        else "k_fused_v<9,TRT,F16Dev> (4 nodes/thread)"),
+                             "kernel": kernels[storage],
                              "kernel_ms_avg": round(kavg16, 5), "bytes_per_launch": bytes16},
             }
             del m16
