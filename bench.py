@@ -348,8 +348,8 @@ def main():
     ap.add_argument("--precision-quadratures", default="9,17,37")
     a = ap.parse_args()
     if a.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
-              "(the lattice shards as y-slabs in design only)", file=sys.stderr)
+        print(f"bench.py: liblbm_hip runs on one GPU; --gpus {a.gpus} is not supported "
+              "(sharding the lattice as y-slabs is designed, not implemented)", file=sys.stderr)
         sys.exit(2)
 
     import lbmhip as L
