@@ -274,6 +274,37 @@ def startup_profile(flow, ny, nu, t, F=1e-6, u_wall=0.05, nterms=4000):
     return u_wall * y / h + (coef * decay) @ modes
 
 
+def mrt_leg(L, steps=20):
+    """Regularised MRT (mrt.jl:56-118) in FP64 on the periodic Taylor-Green
+    vortex, D2Q9 at 4096^2 and D2Q17 / D2Q37 at 8192 x 2048, all relaxation
+    times 0.8, beside SRT(0.8) on the same lattice: kernel time per step from
+    HIP events and the fraction of 8 TB/s at 2 Q 8 bytes per node."""
+    rows = []
+    for qid, nx, ny in ((9, 4096, 4096), (17, 8192, 2048), (37, 8192, 2048)):
+        q = L.Quadratures[f"D2Q{qid}"]()
+        x = (np.arange(nx) + 0.5) * (2 * np.pi / nx)
+        y = (np.arange(ny) + 0.5) * (2 * np.pi / ny)
+        X, Y = np.meshgrid(x, y, indexing="ij")
+        f0 = L.equilibrium_populations(q, np.ones((nx, ny)), 0.02 * np.sin(X) * np.cos(Y),
+                                       -0.02 * np.cos(X) * np.sin(Y))
+        del X, Y
+        N = max(2, L.order(q) // 2)
+        for name, cm in (("MRT", L.MRT(q, [0.8] * N)), ("SRT", L.SRT(0.8))):
+            print(f"[bench] D2Q{qid} {nx}x{ny} FP64 {name}: {steps} steps", file=sys.stderr,
+                  flush=True)
+            m = L.LatticeBoltzmannModel.from_populations(q, f0, cm)
+            L.simulate(m, range(3))
+            ms, nl = L.simulate_timed(m, steps)
+            us = ms / max(nl, 1) * 1e3
+            gbs = 2 * q.Q * 8 * nx * ny / (us * 1e-6) / 1e9
+            rows.append({"quadrature": f"D2Q{qid}", "lattice": [nx, ny], "collision": name,
+                         "us_per_step": round(us, 1), "mlups": round(nx * ny / us, 1),
+                         "achieved_GBps": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)})
+            del m
+        del f0
+    return rows
+
+
 def precision_leg(L, steps, nx=8192, ny=2048, tau=0.8, qids=(9, 17, 37),
                   flows=("poiseuille", "couette"),
                   storages=("f64", "f32", "f16dev", "bf16dev", "f16neq", "bf16neq")):
@@ -346,6 +377,8 @@ def main():
                     help="run only the 8192x2048 channel precision table (minutes)")
     ap.add_argument("--precision-steps", type=int, default=20000)
     ap.add_argument("--precision-quadratures", default="9,17,37")
+    ap.add_argument("--mrt", action="store_true",
+                    help="run only the FP64 MRT-against-SRT table (D2Q9/17/37)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: liblbm_hip runs on one GPU; --gpus {a.gpus} is not supported "
@@ -356,6 +389,9 @@ def main():
     if L.device_count() < 1:
         print("bench.py: no GPU visible", file=sys.stderr)
         sys.exit(1)
+    if a.mrt:
+        print(json.dumps({"mrt": mrt_leg(L)}), flush=True)
+        return
     if a.precision:
         qids = tuple(int(v) for v in a.precision_quadratures.split(","))
         rows = precision_leg(L, a.precision_steps, qids=qids)
