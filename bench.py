@@ -305,6 +305,33 @@ def mrt_leg(L, steps=20):
     return rows
 
 
+def sizes_leg(L, sizes, steps=20):
+    """D2Q9 TRT(0.8, 1.1) on periodic n x n lattices (uniform flow 0.01 along
+    x, equilibrium start) in FP64 and FP32: kernel time per step from HIP
+    events and algorithmic GB/s (2 Q sizeof per node).  Rows of whole 128-byte
+    lines run the row-shaped kernel, other widths the flat-index one."""
+    q = L.D2Q9()
+    rows = []
+    for n in sizes:
+        f0 = L.equilibrium_populations(q, np.ones((n, n)), np.full((n, n), 0.01),
+                                       np.zeros((n, n)))
+        for storage, eb in (("f64", 8), ("f32", 4)):
+            print(f"[bench] D2Q9 {n}x{n} {storage}: {steps} steps", file=sys.stderr, flush=True)
+            m = L.LatticeBoltzmannModel.from_populations(q, f0, L.TRT.fields(0.8, 1.1),
+                                                         storage=storage)
+            L.simulate(m, range(3))
+            ms, nl = L.simulate_timed(m, steps)
+            us = ms / max(nl, 1) * 1e3
+            gbs = 2 * q.Q * eb * n * n / (us * 1e-6) / 1e9
+            rows.append({"lattice": [n, n], "storage": storage,
+                         "kernel": "row-shaped" if (n * eb) % 128 == 0 else "flat-index",
+                         "us_per_step": round(us, 1), "achieved_GBps": round(gbs, 1),
+                         "frac": round(gbs / HBM_PEAK_GBS, 4)})
+            del m
+        del f0
+    return rows
+
+
 def precision_leg(L, steps, nx=8192, ny=2048, tau=0.8, qids=(9, 17, 37),
                   flows=("poiseuille", "couette"),
                   storages=("f64", "f32", "f16dev", "bf16dev", "f16neq", "bf16neq")):
@@ -379,6 +406,8 @@ def main():
     ap.add_argument("--precision-quadratures", default="9,17,37")
     ap.add_argument("--mrt", action="store_true",
                     help="run only the FP64 MRT-against-SRT table (D2Q9/17/37)")
+    ap.add_argument("--sizes", default="",
+                    help="run only D2Q9 TRT FP64/FP32 on these n x n lattices (e.g. 4096,4100)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: liblbm_hip runs on one GPU; --gpus {a.gpus} is not supported "
@@ -391,6 +420,10 @@ def main():
         sys.exit(1)
     if a.mrt:
         print(json.dumps({"mrt": mrt_leg(L)}), flush=True)
+        return
+    if a.sizes:
+        sizes = [int(v) for v in a.sizes.split(",")]
+        print(json.dumps({"sizes": sizes_leg(L, sizes)}), flush=True)
         return
     if a.precision:
         qids = tuple(int(v) for v in a.precision_quadratures.split(","))
