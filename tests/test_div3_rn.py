@@ -95,3 +95,16 @@ def test_div3_rn_equals_ieee_division(tmp_path):
     r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=300)
     n, bad = map(int, r.stdout.split()[-2:])
     assert n > 10 ** 7 and bad == 0, r.stdout
+
+
+@pytest.mark.skipif(shutil.which("gcc") is None, reason="gcc is not installed")
+def test_div3_rn_check_catches_a_missing_correction(tmp_path):
+    """the same check on the plain product x * RN(1/3) (the correction step
+    dropped) reports mismatches, so the check above can fail"""
+    c = tmp_path / "div3_bad.c"
+    exe = tmp_path / "div3_bad"
+    c.write_text(SRC.replace("return fma(rem, r, q);", "return q + 0.0 * rem;"))
+    subprocess.run(["gcc", "-O2", "-ffp-contract=off", str(c), "-o", str(exe), "-lm"], check=True)
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=300)
+    n, bad = map(int, r.stdout.split()[-2:])
+    assert r.returncode != 0 and bad > n // 10, r.stdout
