@@ -310,20 +310,6 @@ __device__ __forceinline__ Mom4 make_rec(float drho, float ux, float uy, const S
     }
 }
 
-// feq_I - w_I rho0 from an equilibrium record
-template <int QID, int I>
-__device__ __forceinline__ float geq_rec(const Mom4 &m, const StepParams<float> &p) {
-    constexpr int FEQ = QT<QID>::FEQ;
-    if constexpr (FEQ <= 2) {
-        const float cA = udot<QID, I>(m.a1, m.a2);
-        if constexpr (FEQ == 2) return p.w[I] * ((m.a0 + cA) + m.a3 * (cA * cA));
-        else return p.w[I] * (m.a0 + cA);
-    } else {
-        const float usq = m.a1 * m.a1 + m.a2 * m.a2;
-        return feq_dev_i<QID, I, float>(m.a0, p.rho0 + m.a0, m.a1, m.a2, usq, p);
-    }
-}
-
 // A node's own shifted equilibrium, shared by all of its populations.  For
 // FEQ >= 3 the expansion of maxwell_boltzmann_equilibrium.jl in t = c_i.u,
 //   s t + (s^2 t^2 - s u^2)/2 + s t (s^2 t^2 - 3 s u^2)/6
@@ -368,20 +354,39 @@ __device__ __forceinline__ NodeEq node_eq(float drho, float ux, float uy,
     return e;
 }
 
+// w_I (k0 + t (k1 + t (k2 + t (k3 + t k4)))) with t = c_I.u (FEQ >= 3)
 template <int QID, int I>
-__device__ __forceinline__ float geq_node(const NodeEq &e, const StepParams<float> &p) {
+__device__ __forceinline__ float horner_geq(const NodeEq &e, const StepParams<float> &p) {
+    const float t = udot<QID, I>(e.ux, e.uy);
+    float h = e.k3;
+    if constexpr (QT<QID>::FEQ >= 4) h = h + t * e.k4;
+    h = e.k2 + t * h;
+    h = e.k1 + t * h;
+    h = e.k0 + t * h;
+    return p.w[I] * h;
+}
+
+// feq_I - w_I rho0 from an equilibrium record.  For FEQ >= 3 the record's
+// (drho, u) go through the same per-node coefficients as a node's own
+// equilibrium: the writer of a record (the non-equilibrium parts are taken
+// against it) forms them once for all Q populations, and every reader
+// evaluates the identical expression.
+template <int QID, int I>
+__device__ __forceinline__ float geq_rec(const Mom4 &m, const StepParams<float> &p) {
     constexpr int FEQ = QT<QID>::FEQ;
     if constexpr (FEQ <= 2) {
-        return geq_rec<QID, I>(e.rec, p);
+        const float cA = udot<QID, I>(m.a1, m.a2);
+        if constexpr (FEQ == 2) return p.w[I] * ((m.a0 + cA) + m.a3 * (cA * cA));
+        else return p.w[I] * (m.a0 + cA);
     } else {
-        const float t = udot<QID, I>(e.ux, e.uy);
-        float h = e.k3;
-        if constexpr (FEQ >= 4) h = h + t * e.k4;
-        h = e.k2 + t * h;
-        h = e.k1 + t * h;
-        h = e.k0 + t * h;
-        return p.w[I] * h;
+        return horner_geq<QID, I>(node_eq<QID>(m.a0, m.a1, m.a2, p), p);
     }
+}
+
+template <int QID, int I>
+__device__ __forceinline__ float geq_node(const NodeEq &e, const StepParams<float> &p) {
+    if constexpr (QT<QID>::FEQ <= 2) return geq_rec<QID, I>(e.rec, p);
+    else return horner_geq<QID, I>(e, p);
 }
 
 // shifted moments: drho = sum g, rho u = sum c g (sum_i c_i w_i = 0)
