@@ -203,36 +203,64 @@ function apply!(bcs::AbstractVector{<:BoundaryCondition}, q::Quadrature, f_new::
     nothing
 end
 
-# simulate(model, time) (lattice_boltzmann_model.jl:60-77) with device arrays:
-# runs of steps between processing points go to lbm_simulate (one fused
-# collide-stream kernel per step); next! receives a host copy of f_stream at
-# the steps where the processing method does work (every step when it
-# processes, otherwise every 100th step and the last one).
+# Steps t at which next!(pm, q, f, t) reads the populations.  At every other
+# step it returns false without touching them, so simulate may skip the call:
+# TrackHydrodynamicErrors checks its stop criteria at multiples of 100 and
+# records a row at n_steps, or at every step with should_process
+# (track_hydrodynamic_errors.jl:52-66); any other processing method is called
+# at every step.
+function reads_state(pm, t)
+    if pm isa LatticeBoltzmann.TrackHydrodynamicErrors
+        return pm.should_process || mod(t, 100) == 0 || t == pm.n_steps
+    end
+    return true
+end
+
+processing_view(pm, f::HipLattice) =
+    pm isa LatticeBoltzmann.TrackHydrodynamicErrors ? f : Array(f)
+
+# simulate(model, time) (lattice_boltzmann_model.jl:60-77) with device arrays.
+# Step t collides and applies the BCs at time t * Δt, Δt = delta_t(problem) of
+# the processing method (0 without one), and is followed by next!(model, t + 1);
+# the loop ends with the reference's closing next!(model, last(time) + 1).
+# Runs of steps between the points where next! reads the state go to one
+# lbm_simulate call (one fused collide-stream kernel per step).  A model with
+# a force closure advances one step per call instead, the closure evaluated at
+# that step's time, as the reference's per-node force(x, y, t * Δt)
+# (srt.jl:52, trt.jl:77, mrt.jl:92) is.
 function simulate(model::LatticeBoltzmannModel{<:Quadrature, <:HipLattice}, time)
     pm = model.processing_method
-    every = hasproperty(pm, :should_process) && pm.should_process ? 1 : 100
-    m = native_model(model.quadrature)
+    q = model.quadrature
+    Δt = isdefined(pm, :problem) ? LatticeBoltzmann.delta_t(pm.problem) : 0.0
+    m = native_model(q)
     cm = model.collision_model
     set_collision!(m, cm)
     nx, ny, _ = size(model.f_stream)
     set_bcs!(m, model.boundary_conditions)
-    t = first(time)
-    while t <= last(time)
-        stop = min(last(time), (div(t + 1, every) + 1) * every - 1)
-        n = stop - t + 1
-        set_force!(m, cm, nx, ny, t)  # constant-in-time forces only across a run
+    forced = hasproperty(cm, :force) && cm.force !== nothing
+    forced || set_force!(m, cm, nx, ny, 0.0)
+    n = length(time)
+    i = 1
+    while i <= n
+        j = i
+        if !forced
+            while j < n && !reads_state(pm, time[j] + 1)
+                j += 1
+            end
+        else
+            set_force!(m, cm, nx, ny, time[i] * Δt)
+        end
         check(ccall((:lbm_simulate, libpath), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Int64),
-                    m.handle, model.f_stream.handle, model.f_collision.handle, n),
+                    m.handle, model.f_stream.handle, model.f_collision.handle, j - i + 1),
               "lbm_simulate")
-        t = stop + 1
-        # TrackHydrodynamicErrors reads the device array (methods below); any
-        # other processing method gets a host copy
-        f = pm isa LatticeBoltzmann.TrackHydrodynamicErrors ? model.f_stream :
-            Array(model.f_stream)
-        if LatticeBoltzmann.next!(pm, model.quadrature, f, t)
+        t_next = time[j] + 1
+        if reads_state(pm, t_next) &&
+           LatticeBoltzmann.next!(pm, q, processing_view(pm, model.f_stream), t_next)
             return model
         end
+        i = j + 1
     end
+    LatticeBoltzmann.next!(pm, q, processing_view(pm, model.f_stream), last(time) + 1)
     model
 end
 
