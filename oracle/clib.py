@@ -75,6 +75,10 @@ def lib() -> ctypes.CDLL:
         _lib.lbo_velocity.argtypes = [qp, dp, ctypes.c_double, dp]
         _lib.lbo_equilibrium.argtypes = [qp, ctypes.c_double, dp, dp]
         _lib.lbo_hermite_equilibrium.argtypes = [qp, ctypes.c_double, dp, dp]
+        _lib.lbo_init_node.argtypes = [qp, ctypes.c_int, ctypes.c_double, dp, ctypes.c_double, dp,
+                                       ctypes.c_double, ctypes.c_double, dp]
+        _lib.lbo_initialize.argtypes = [qp, ctypes.c_int, ctypes.c_int, ctypes.c_int, dp, dp, dp,
+                                        dp, dp, ctypes.c_double, ctypes.c_double, dp]
         i = ctypes.c_int
         d = ctypes.c_double
         _lib.lbo_collide_srt.argtypes = [qp, i, i, dp, dp, d, i, dp]
@@ -155,6 +159,37 @@ def hermite_equilibrium(qid, rho, u):
     uu = np.ascontiguousarray(u, dtype=np.float64)
     out = np.zeros(q.Q)
     lib().lbo_hermite_equilibrium(ctypes.byref(q), float(rho), _dp(uu), _dp(out))
+    return out
+
+
+def init_node(qid, mode, rho, u, T=1.0, G=None, tau=0.0, factor=1.0):
+    """One node of the analytic initial conditions (lbo_init_node)."""
+    q = quadrature(qid)
+    uu = np.ascontiguousarray(u, dtype=np.float64)
+    g = np.ascontiguousarray(np.zeros(4) if G is None else np.asarray(G, dtype=np.float64)
+                             .reshape(2, 2).ravel(order="F"))
+    out = np.zeros(q.Q)
+    lib().lbo_init_node(ctypes.byref(q), int(mode), float(rho), _dp(uu), float(T), _dp(g),
+                        float(tau), float(factor), _dp(out))
+    return out
+
+
+def initialize(qid, mode, rho, ux, uy, T=None, G=None, tau=0.0, factor=1.0):
+    """initialize(strategy, q, problem) for the analytic strategies over a
+    lattice: planes (nx, ny); G (nx, ny, 2, 2) with G[..., a, b]; returns
+    (nx, ny, Q) column-major."""
+    q = quadrature(qid)
+    rho = np.asfortranarray(rho, dtype=np.float64)
+    nx, ny = rho.shape
+    ux = np.asfortranarray(ux, dtype=np.float64)
+    uy = np.asfortranarray(uy, dtype=np.float64)
+    Tp = None if T is None else np.asfortranarray(T, dtype=np.float64)
+    Gp = None if G is None else np.asfortranarray(
+        np.asarray(G, dtype=np.float64).reshape(nx, ny, 4, order="F"))
+    out = np.zeros((nx, ny, q.Q), order="F")
+    lib().lbo_initialize(ctypes.byref(q), nx, ny, int(mode), _dp(rho), _dp(ux), _dp(uy),
+                         None if Tp is None else _dp(Tp), None if Gp is None else _dp(Gp),
+                         float(tau), float(factor), _dp(out))
     return out
 
 

@@ -345,6 +345,115 @@ void lbo_hermite_equilibrium(const lbo_quad *q, double rho, const double *u, dou
     }
 }
 
+/* equilibrium_coefficient(Val{n}, q, rho, u, T) with the temperature terms
+ * (velocity_distribution_function/hermite.jl:37-77), entry k of the
+ * column-major tensor (first index fastest).  Val{2}: u u' + cs (T - 1) I adds
+ * to the diagonal only.  Val{4} keeps the reference's indices as written:
+ * its third and fifth (T - 1) terms pair u_a u_d and u_b u_d with delta(b, d)
+ * and delta(a, d). */
+static void eq_coefficient_T(int n, double rho, const double *u, double T, double cs,
+                             double *a) {
+    const double ct = cs * (T - 1.0);
+    if (n == 1) {
+        a[0] = rho * u[0];
+        a[1] = rho * u[1];
+        return;
+    }
+    for (int k = 0; k < (1 << n); ++k) {
+        const int ia = k & 1, ib = (k >> 1) & 1, ic = (k >> 2) & 1, id = (k >> 3) & 1;
+        double v;
+        if (n == 2) {
+            v = u[ia] * u[ib];
+            if (ia == ib) v = v + ct;
+        } else if (n == 3) {
+            const double S = (u[ia] * (double)delta(ib, ic) + u[ib] * (double)delta(ia, ic)) +
+                             u[ic] * (double)delta(ia, ib);
+            v = ((u[ia] * u[ib]) * u[ic]) + ct * S;
+        } else {
+            const double S1 =
+                ((((u[ia] * u[ib]) * (double)delta(ic, id) + (u[ia] * u[ic]) * (double)delta(ib, id)) +
+                  (u[ia] * u[id]) * (double)delta(ib, id)) +
+                 (u[ib] * u[ic]) * (double)delta(ia, id)) +
+                (u[ib] * u[id]) * (double)delta(ia, id);
+            const double S1b = S1 + (u[ic] * u[id]) * (double)delta(ia, ib);
+            const int S0 = (delta(ia, ib) * delta(ic, id) + delta(ia, ic) * delta(ib, id)) +
+                           delta(ia, id) * delta(ib, ic);
+            v = ((((u[ia] * u[ib]) * u[ic]) * u[id]) + ct * S1b) +
+                ((cs * cs) * ((T - 1.0) * (T - 1.0))) * (double)S0;
+        }
+        a[k] = rho * v;
+    }
+}
+
+/* initial_condition(strategy, q, problem, x, y) of one node, for the
+ * analytic strategies:
+ *   mode 0, AnalyticalEquilibrium (initial_conditions/analytical_equilibrium.jl:
+ *     8-17): equilibrium(q, rho, u, T) = hermite_based_equilibrium!
+ *     (hermite.jl:10-33, maxwell_boltzmann_equilibrium.jl:1-10), the Hermite
+ *     sum w_i (rho + sum_n a_eq[n] : H_n(c_i) / (n! cs^n)) at temperature T;
+ *   mode 1, AnalyticalEquilibriumAndOffEquilibrium for the Taylor-Green vortex
+ *     (analytical_offequilibrium.jl:50-87): minus
+ *     w_i (s tau_eff rho 1) / 2 * H_2(c_i) : (G + G'), tau_eff = tau + 1/2;
+ *   mode 2, the same for any other problem (analytical_offequilibrium.jl:10-48):
+ *     plus -factor w_i 0.5 ((tau + 0.5) s) H_2(c_i) : (G + G').
+ * G is the 2 x 2 velocity gradient as the reference passes it (column-major,
+ * G[a + 2 b]); tau = s nu, the reference's tau before the 1/2.  cs^n comes
+ * from libm pow, as Julia's Float64 ^ Int; dot is a left fold in column-major
+ * order. */
+void lbo_init_node(const lbo_quad *q, int mode, double rho, const double *u, double T,
+                   const double *G, double tau, double factor, double *f) {
+    const int N = q->order / 2;
+    const double cs = 1.0 / q->s;
+    double aeq[4][16];
+    for (int n = 1; n <= N; ++n) eq_coefficient_T(n, rho, u, T, cs, aeq[n - 1]);
+    double S[4];
+    if (mode != 0)
+        for (int b = 0; b < 2; ++b)
+            for (int a = 0; a < 2; ++a) S[a + 2 * b] = G[a + 2 * b] + G[b + 2 * a];
+    for (int i = 0; i < q->Q; ++i) {
+        double sum = 0.0;
+        for (int n = 1; n <= N; ++n) {
+            double H[16];
+            hermite_tensor(n, q->cx[i], q->cy[i], cs, H);
+            double d = 0.0;
+            for (int k = 0; k < (1 << n); ++k) d = (k == 0) ? aeq[n - 1][k] * H[k] : d + aeq[n - 1][k] * H[k];
+            const double term = d / ((double)factorial_int(n) * pow(cs, (double)n));
+            sum = (n == 1) ? term : sum + term;
+        }
+        double fi = q->w[i] * (rho + sum);
+        if (mode != 0) {
+            double H2[4];
+            hermite_tensor(2, q->cx[i], q->cy[i], cs, H2);
+            double d2 = H2[0] * S[0];
+            for (int k = 1; k < 4; ++k) d2 = d2 + H2[k] * S[k];
+            if (mode == 1)
+                fi = fi - ((q->w[i] * (((q->s * (tau + 0.5)) * rho) * 1.0)) / 2.0) * d2;
+            else
+                fi = fi + ((((-factor) * q->w[i]) * 0.5) * ((tau + 0.5) * q->s)) * d2;
+        }
+        f[i] = fi;
+    }
+}
+
+/* initialize(strategy, q, problem) over a lattice (initial_conditions.jl:7-22):
+ * planes rho, ux, uy, T (NULL: 1) and G (4 planes, NULL for mode 0), each
+ * (nx, ny) column-major; f is (nx, ny, Q). */
+void lbo_initialize(const lbo_quad *q, int nx, int ny, int mode, const double *rho,
+                    const double *ux, const double *uy, const double *T, const double *G,
+                    double tau, double factor, double *f) {
+    const long plane = (long)nx * ny;
+    double fn[37];
+#pragma omp parallel for private(fn) schedule(static)
+    for (long n = 0; n < plane; ++n) {
+        const double u[2] = {ux[n], uy[n]};
+        double g[4] = {0.0, 0.0, 0.0, 0.0};
+        if (G)
+            for (int k = 0; k < 4; ++k) g[k] = G[k * plane + n];
+        lbo_init_node(q, mode, rho[n], u, T ? T[n] : 1.0, g, tau, factor, fn);
+        for (int i = 0; i < q->Q; ++i) f[i * plane + n] = fn[i];
+    }
+}
+
 /* ------------------------------------------------------------------------ */
 /* Collision operators                                                       */
 /* ------------------------------------------------------------------------ */

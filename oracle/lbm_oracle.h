@@ -51,6 +51,13 @@ double lbo_density(const lbo_quad *q, const double *f);
 void lbo_velocity(const lbo_quad *q, const double *f, double rho, double *u);
 void lbo_equilibrium(const lbo_quad *q, double rho, const double *u, double *feq);
 void lbo_hermite_equilibrium(const lbo_quad *q, double rho, const double *u, double *feq);
+/* analytic initial conditions (mode 0: AnalyticalEquilibrium, 1: equilibrium +
+ * off-equilibrium of the Taylor-Green vortex, 2: the same for other problems) */
+void lbo_init_node(const lbo_quad *q, int mode, double rho, const double *u, double T,
+                   const double *G, double tau, double factor, double *f);
+void lbo_initialize(const lbo_quad *q, int nx, int ny, int mode, const double *rho,
+                    const double *ux, const double *uy, const double *T, const double *G,
+                    double tau, double factor, double *f);
 
 /* force: kind NONE (ptr ignored), CONST (2 doubles), FIELD ((nx,ny,2) col-major) */
 void lbo_collide_srt(const lbo_quad *q, int nx, int ny, const double *fin, double *fout,
