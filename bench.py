@@ -43,16 +43,60 @@ def tgv_populations(L, q, n, u0=0.02):
     return L.equilibrium_populations(q, rho, ux, uy)
 
 
+# the Taylor-Green workload's length: the accuracy of every storage is taken
+# after this many steps, whatever --steps is
+ACCURACY_STEPS = 5000
+# fused launches timed one by one for the spread of the kernel's duration
+SPREAD_LAUNCHES = 200
+
+
 def run_storage(L, q, f0, cm, storage, steps, warmup):
+    """W untimed warm-up steps, then K timed steps (wall clock around them,
+    HIP events around the fused launches), with the shader clock sampled
+    before and after the timed steps."""
     model = L.LatticeBoltzmannModel.from_populations(q, f0, cm, storage=storage)
     if warmup:
         L.simulate(model, range(warmup))
     L.synchronize()
+    c0 = L.clock_sample()
     t0 = time.perf_counter()
     kernel_ms, nlaunch = L.simulate_timed(model, steps)
     L.synchronize()
     wall = time.perf_counter() - t0
-    return model, wall, kernel_ms, nlaunch
+    c1 = L.clock_sample()
+    return model, wall, kernel_ms, nlaunch, L.average_clock_mhz(c0, c1)
+
+
+def launch_spread(L, model, launches=SPREAD_LAUNCHES):
+    """Per-launch duration of the fused kernel over a run of ``launches``
+    launches (untimed for the bench line; HIP events between launches), with
+    the average shader clock over that run."""
+    c0 = L.clock_sample()
+    t = L.simulate_launch_times(model, launches + 1) * 1e3
+    c1 = L.clock_sample()
+    return {"launches": int(t.size), "us_min": round(float(t.min()), 2),
+            "us_avg": round(float(t.mean()), 2), "us_max": round(float(t.max()), 2),
+            "us_first_20_avg": round(float(t[:20].mean()), 2),
+            "us_last_20_avg": round(float(t[-20:].mean()), 2),
+            "shader_clock_mhz": _r(L.average_clock_mhz(c0, c1))}
+
+
+def _r(v, nd=1):
+    return None if v is None else round(v, nd)
+
+
+def velocity_after(L, q, f0, cm, storage, steps):
+    """The velocity field after ``steps`` steps (untimed)."""
+    model = L.LatticeBoltzmannModel.from_populations(q, f0, cm, storage=storage)
+    L.simulate(model, range(steps))
+    m = L.moments(model.f_stream, stress=False)
+    del model
+    return m.ux, m.uy
+
+
+def rel_l2(a, b):
+    num = np.sqrt(np.sum((a[0] - b[0]) ** 2 + (a[1] - b[1]) ** 2))
+    return float(num / np.sqrt(np.sum(b[0] ** 2 + b[1] ** 2)))
 
 
 def traffic_for(key):
@@ -63,12 +107,43 @@ def traffic_for(key):
         return None
 
 
+def host_cpus():
+    """CPUs this process may run on: the affinity mask, capped by a cgroup
+    CPU quota when one is set; with the host's logical CPU count and model."""
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    quota = None
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as fh:
+            q, per = fh.read().split()[:2]
+            if q != "max":
+                quota = max(1, int(int(q) / int(per)))
+    except (OSError, ValueError):
+        pass
+    model = None
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    threads = min(usable, quota) if quota else usable
+    return {"threads": threads, "nproc": os.cpu_count(), "affinity": usable,
+            "cgroup_quota_cpus": quota, "model": model}
+
+
 def cpu_baseline(n, tau_s, tau_a, f0, target_s):
     """The FP64 CPU restatement (the oracle) timed on this host's cores on a
     bounded sample of the same workload: as many steps of the 4096^2 lattice
-    as fit in about target_s seconds (at most 400)."""
+    as fit in about target_s seconds (at most 400), on every CPU the process
+    may use, and the same on one core."""
     from oracle import clib
-    threads = min(os.cpu_count() or 1, 16)
+    host = host_cpus()
+    threads = host["threads"]
     clib.set_threads(threads)
     S = np.asfortranarray(f0.copy())
     C = np.asfortranarray(f0.copy())
@@ -93,6 +168,7 @@ def cpu_baseline(n, tau_s, tau_a, f0, target_s):
     return {"value": round(mlups, 2), "unit": "MLUPS", "cores": threads, "kind": "port",
             "sample": f"D2Q9 TRT Taylor-Green {n}x{n}, FP64, {steps} steps "
                       f"({dt:.1f} s) with the C restatement on {threads} OpenMP threads",
+            "host": host,
             "single_core": {"value": round(n * n * steps1 / dt1 / 1e6, 2), "unit": "MLUPS",
                             "sample": f"{steps1} steps ({dt1:.1f} s) on 1 thread"}}
 
@@ -334,7 +410,7 @@ def sizes_leg(L, sizes, steps=20):
 
 def precision_leg(L, steps, nx=8192, ny=2048, tau=0.8, qids=(9, 17, 37),
                   flows=("poiseuille", "couette"),
-                  storages=("f64", "f32", "f16dev", "bf16dev", "f16neq", "bf16neq")):
+                  storages=("f64", "f32", "f16dev", "bf16dev", "f16neq", "bf16neq"), force=1e-6):
     """D2Q9 / D2Q17 / D2Q37 SRT(tau) on an nx x ny channel (periodic in x,
     walls at the bottom and top rows), from rest: Poiseuille with the constant
     body force (1e-6, 0) and bounce-back walls (poiseuille.jl:93-96), Couette
@@ -351,13 +427,13 @@ def precision_leg(L, steps, nx=8192, ny=2048, tau=0.8, qids=(9, 17, 37),
         for flow in flows:
             full = ((1, nx), (1, ny))
             if flow == "poiseuille":
-                cm = L.SRT(tau, np.array([1e-6, 0.0]))
+                cm = L.SRT(tau, np.array([force, 0.0]))
                 bcs = [L.BounceBack(L.North(), *full), L.BounceBack(L.South(), *full)]
             else:
                 cm = L.SRT(tau)
                 bcs = [L.BounceBack(L.South(), *full),
                        L.MovingWall(L.North(), *full, (0.05, 0.0))]
-            ua = startup_profile(flow, ny, nu, steps)
+            ua = startup_profile(flow, ny, nu, steps, F=force)
             ref = None
             for st in storages:
                 m = L.LatticeBoltzmannModel.from_populations(q, f0, cm, bcs, storage=st)
@@ -404,6 +480,11 @@ def main():
                     help="run only the 8192x2048 channel precision table (minutes)")
     ap.add_argument("--precision-steps", type=int, default=20000)
     ap.add_argument("--precision-quadratures", default="9,17,37")
+    ap.add_argument("--precision-storages", default="f64,f32,f16dev,bf16dev,f16neq,bf16neq",
+                    help="storages of the channel table (f64 first: the reference run)")
+    ap.add_argument("--precision-force", type=float, default=1e-6,
+                    help="body force of the Poiseuille channel")
+    ap.add_argument("--precision-flows", default="poiseuille,couette")
     ap.add_argument("--mrt", action="store_true",
                     help="run only the FP64 MRT-against-SRT table (D2Q9/17/37)")
     ap.add_argument("--sizes", default="",
@@ -427,7 +508,10 @@ def main():
         return
     if a.precision:
         qids = tuple(int(v) for v in a.precision_quadratures.split(","))
-        rows = precision_leg(L, a.precision_steps, qids=qids)
+        rows = precision_leg(L, a.precision_steps, qids=qids,
+                             storages=tuple(a.precision_storages.split(",")),
+                             force=a.precision_force,
+                             flows=tuple(a.precision_flows.split(",")))
         print(json.dumps({"precision": rows}), flush=True)
         return
     n, K, W = a.n, a.steps, a.warmup
@@ -439,12 +523,15 @@ def main():
     nodes = n * n
 
     print(f"[bench] FP64: {W} warmup + {K} timed steps", file=sys.stderr, flush=True)
-    m64, wall64, kms64, nl64 = run_storage(L, q, f0, cm, "f64", K, W)
+    m64, wall64, kms64, nl64, clk64 = run_storage(L, q, f0, cm, "f64", K, W)
     mlups64 = nodes * K / wall64 / 1e6
     bytes64 = 2 * q.Q * 8 * nodes
     kavg64 = kms64 / max(nl64, 1)
     gbs64 = bytes64 / (kavg64 * 1e-3) / 1e9
-    u64 = L.moments(m64.f_stream, stress=False)
+    spread64 = launch_spread(L, m64)
+    print(f"[bench] FP64 accuracy reference: {ACCURACY_STEPS} untimed steps", file=sys.stderr,
+          flush=True)
+    u64 = velocity_after(L, q, f0, cm, "f64", ACCURACY_STEPS)
 
     result = {
         "metric": METRIC,
@@ -479,6 +566,8 @@ def main():
             "kernel": "k_fused_v<9,TRT,F64> (128x2 blocks, 1 node/thread, non-temporal stores)",
             "kernel_ms_avg": round(kavg64, 5),
             "bytes_per_launch": bytes64,
+            "shader_clock_mhz": _r(clk64),
+            "spread": spread64,
         },
     }
 
@@ -502,33 +591,36 @@ def main():
                                        ("f32", "fp32", 4 * q.Q)):
             print(f"[bench] {storage}: {W} warmup + {K} timed steps", file=sys.stderr,
                   flush=True)
-            m16, wall16, kms16, nl16 = run_storage(L, q, f0, cm, storage, K, W)
+            m16, wall16, kms16, nl16, clk16 = run_storage(L, q, f0, cm, storage, K, W)
             mlups16 = nodes * K / wall16 / 1e6
             bytes16 = 2 * per_node * nodes
             kavg16 = kms16 / max(nl16, 1)
             gbs16 = bytes16 / (kavg16 * 1e-3) / 1e9
-            u16 = L.moments(m16.f_stream, stress=False)
-            num = np.sqrt(np.sum((u16.ux - u64.ux) ** 2 + (u16.uy - u64.uy) ** 2))
-            den = np.sqrt(np.sum(u64.ux ** 2 + u64.uy ** 2))
+            spread16 = launch_spread(L, m16)
+            del m16
+            print(f"[bench] {storage} accuracy: {ACCURACY_STEPS} untimed steps", file=sys.stderr,
+                  flush=True)
+            u16 = velocity_after(L, q, f0, cm, storage, ACCURACY_STEPS)
             result[key] = {
                 "value": round(mlups16, 1),
                 "unit": "MLUPS",
                 "storage": storage,
                 "ratio_vs_f64": round(mlups16 / mlups64, 3),
-                "rel_l2_velocity_err_vs_f64": float(num / den),
-                "steps_compared": W + K,
+                "rel_l2_velocity_err_vs_f64": rel_l2(u16, u64),
+                "steps_compared": ACCURACY_STEPS,
                 "ms_per_step": round(wall16 / K * 1e3, 5),
                 "roofline": {"bound": "hbm", "achieved": round(gbs16, 1), "peak": HBM_PEAK_GBS,
                              "unit": "GB/s", "frac": round(gbs16 / HBM_PEAK_GBS, 4),
                              "traffic": traffic_for(f"d2q9_trt_{storage}_4096"),
                              "kernel": kernels[storage],
-                             "kernel_ms_avg": round(kavg16, 5), "bytes_per_launch": bytes16},
+                             "kernel_ms_avg": round(kavg16, 5), "bytes_per_launch": bytes16,
+                             "shader_clock_mhz": _r(clk16), "spread": spread16},
             }
-            del m16
 
     if not a.no_processing:
         print("[bench] on-device processing row", file=sys.stderr, flush=True)
-        result["processing"] = processing_leg(L, q, m64, n, 0.02, cm.τ_symmetric, W + K)
+        result["processing"] = processing_leg(L, q, m64, n, 0.02, cm.τ_symmetric,
+                                              W + K + SPREAD_LAUNCHES + 1)
     del m64
     if not a.no_batch:
         print("[bench] batch of 512 lattices", file=sys.stderr, flush=True)

@@ -41,7 +41,13 @@ enum { LBM_D2Q4 = 4, LBM_D2Q5 = 5, LBM_D2Q9 = 9, LBM_D2Q13 = 13,
  *             coefficients (from rho - rho0 and u) and, per population, the
  *             non-equilibrium part f_i - feq_i(record) in binary16 (scaled by 2^16);
  *   BF16NEQ   the same with bfloat16 non-equilibrium parts (unscaled).
- * F32 / *DEV / *NEQ arithmetic runs on g_i = f_i - w_i rho0 (rho0 of the field). */
+ * F32 / *DEV / *NEQ arithmetic runs on g_i = f_i - w_i rho0 (rho0 of the field).
+ * The DEV storages round the whole deviation f_i - w_i rho0 to 16 bits: a flow
+ * driven by a small body force or a moving wall stops accelerating once the
+ * per-step change of a population falls below half a 16-bit ulp of its
+ * deviation (measured: 9e-2 to 9.6e-1 relative velocity error on 8192 x 2048
+ * Poiseuille / Couette channels after 20000 steps).  They are unsuitable for
+ * forced or wall-driven flows; use F32 or a NEQ storage there. */
 enum { LBM_STORE_F64 = 0, LBM_STORE_F32 = 1, LBM_STORE_F16DEV = 2, LBM_STORE_BF16DEV = 3,
        LBM_STORE_F16NEQ = 4, LBM_STORE_BF16NEQ = 5 };
 
@@ -78,6 +84,14 @@ int lbm_device_count(void);
 /* Blocks until all work issued by the library has finished. */
 int lbm_synchronize(void);
 
+/* Measurement support; no counterpart in the reference.  Runs a probe on
+ * the library's stream, after the work already issued on it, and returns
+ * out[3 * 64]: per probe block (XCC id, shader-clock cycles, constant-rate
+ * ticks); *tick_mhz (may be NULL) is the constant counter's rate.  Two
+ * samples around a stretch of device work give the average shader clock of
+ * each XCD over it.  Synchronous. */
+int lbm_clock_sample(uint64_t *out, double *tick_mhz);
+
 /* Quadrature constants as the library holds them (c as doubles, 2*Q).
  * Replaces field access q.abscissae / q.weights / q.speed_of_sound_squared /
  * order(q) / opposite(q, i) (src/quadratures.jl:11-19, quadratures/D2Q*.jl). */
@@ -98,6 +112,31 @@ int lbm_field_download(lbm_field f, double *host);
 int lbm_field_copy(lbm_field dst, lbm_field src);
 /* f[x, y, :] for 1-based (x, y): the Q populations of one node in FP64 */
 int lbm_field_node(lbm_field f, int x, int y, double *out);
+
+/* Analytic initial conditions, computed on the device: initialize(strategy,
+ * q, problem) (initial_conditions.jl:7-22) for the analytic strategies, with
+ * the problem's fields evaluated by the caller (the reference's per-node
+ * lattice_density / lattice_velocity / lattice_temperature /
+ * velocity_gradient calls, problems.jl:98-106) and passed as Float64 (nx, ny)
+ * column-major planes:
+ *   LBM_INIT_EQUILIBRIUM  AnalyticalEquilibrium
+ *       (initial_conditions/analytical_equilibrium.jl:8-17): the Hermite-sum
+ *       equilibrium of hermite_based_equilibrium! (hermite.jl:10-33) at
+ *       temperature T;
+ *   LBM_INIT_OFFEQ_TGV    AnalyticalEquilibriumAndOffEquilibrium for the
+ *       Taylor-Green vortex (analytical_offequilibrium.jl:50-87): minus
+ *       w_i (s (tau + 1/2) rho) / 2 * H_2(c_i) : (G + G');
+ *   LBM_INIT_OFFEQ        the same for any other problem (:10-48): plus
+ *       -factor w_i 0.5 ((tau + 0.5) s) H_2(c_i) : (G + G'), factor = the
+ *       domain area.
+ * T may be NULL (T = 1); grad = 4 planes of the 2 x 2 gradient as the
+ * reference passes it (G[a + 2 b] column-major; the generic strategy's
+ * u_max^2-scaled one); tau = s nu, the reference's tau before the 1/2.  The
+ * result is stored in f's storage. */
+enum { LBM_INIT_EQUILIBRIUM = 0, LBM_INIT_OFFEQ_TGV = 1, LBM_INIT_OFFEQ = 2 };
+int lbm_initialize_analytical(lbm_field f, int strategy, const double *rho, const double *ux,
+                              const double *uy, const double *T, const double *grad, double tau,
+                              double factor);
 
 /* ---- model (collision model, force, boundary conditions) --------------- */
 int lbm_model_create(lbm_model *out, int quadrature);
@@ -157,6 +196,12 @@ int lbm_simulate(lbm_model m, lbm_field f_stream, lbm_field f_collision, int64_t
  * the library stream): *ms_total over all nsteps, *nlaunch fused launches. */
 int lbm_simulate_timed(lbm_model m, lbm_field f_stream, lbm_field f_collision, int64_t nsteps,
                        double *ms_total, int64_t *nlaunch);
+/* Measurement support; no counterpart in the reference.  As lbm_simulate,
+ * with a HIP event before every fused launch: launch_ms[k] (nsteps - 1
+ * entries) is the time from the start of fused launch k to the start of the
+ * next (the last: to the end of the last), *nlaunch the number of entries. */
+int lbm_simulate_launch_times(lbm_model m, lbm_field f_stream, lbm_field f_collision,
+                              int64_t nsteps, double *launch_ms, int64_t *nlaunch);
 
 /* ---- batches of independent lattices ----------------------------------- */
 /* Many independent periodic lattices (uneven sizes and relaxation times)
@@ -192,6 +237,29 @@ int lbm_batch_autotune(lbm_batch b, int trial_steps, int *block, int *grid, doub
  * steps from one captured hipGraph (0: direct launches).  *ms_fused (may be
  * NULL) receives the time of the fused steps (HIP events). */
 int lbm_batch_simulate(lbm_batch b, int64_t nsteps, int graph_chunk, double *ms_fused);
+
+/* ---- snapshots ----------------------------------------------------------- */
+/* Storage for TakeSnapshots (take_snapshots.jl:3-29: snapshot = copy(f_in) at
+ * the chosen steps), taken without stalling the step: each snapshot is
+ * decoded / copied into one of device_slots staging buffers in HBM (which
+ * bound the device memory held: device_slots x nx x ny x Q x 8 bytes) on the
+ * library stream; a second HIP stream copies it into pinned host memory, where
+ * every snapshot stays as an (nx, ny, Q) column-major Float64 array.  With a
+ * directory (else NULL) each snapshot is also written there as
+ * snapshot_<t, 10 digits>.npy (NumPy format 1.0: '<f8', Fortran order,
+ * shape (nx, ny, Q)).  The library stream waits only to reuse a staging
+ * buffer whose host copy is still running.  Plotting (take_snapshots.jl:
+ * 31-202) is not done. */
+typedef struct lbm_snapshots_s *lbm_snapshots;
+int lbm_snapshots_create(lbm_snapshots *out, int quadrature, int nx, int ny, int device_slots,
+                         const char *directory);
+int lbm_snapshots_destroy(lbm_snapshots s);
+/* enqueue copy(f) as the snapshot of step t; returns without waiting */
+int lbm_snapshot_take(lbm_snapshots s, lbm_field f, int64_t t);
+/* wait for every snapshot taken so far (and its file); *n (may be NULL) = count */
+int lbm_snapshots_wait(lbm_snapshots s, int *n);
+/* snapshot k, in the order taken: its step and (host may be NULL) its data */
+int lbm_snapshot_get(lbm_snapshots s, int k, int64_t *t, double *host);
 
 /* ---- reverse mode (gradients) ------------------------------------------ */
 /* Differentiable run of a periodic, unforced SRT or TRT model:

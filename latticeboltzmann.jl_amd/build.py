@@ -53,7 +53,17 @@ def _units():
             units.append((name, [f"-ffp-contract={contract}"]))
     return units
 
-HEADERS = ["lbm_quad.h", "lbm_internal.h", "lbm_kernels.h", "lbm_adjoint.h"]
+def _deps(src):
+    """Headers a unit includes (directly or through lbm_kernels.h)."""
+    kern = [os.path.join(CSRC, h) for h in ("lbm_quad.h", "lbm_internal.h", "lbm_kernels.h")]
+    if src == "lbm_capi.cpp":
+        return [os.path.join(CSRC, "lbm_internal.h"), os.path.join(CSRC, "lbm_quad.h"),
+                os.path.join(REPO, "include", "lbm_hip.h")]
+    if src == "lbm_dispatch.cpp":
+        return [os.path.join(CSRC, "lbm_internal.h")]
+    if src.startswith("k_adjoint"):
+        return kern + [os.path.join(CSRC, "lbm_adjoint.h")]
+    return kern
 
 
 def _mtime(p):
@@ -63,14 +73,13 @@ def _mtime(p):
 def build(force: bool = False, verbose: bool = False) -> str:
     os.makedirs(BUILD, exist_ok=True)
     os.makedirs(LIBDIR, exist_ok=True)
-    deps = [os.path.join(CSRC, h) for h in HEADERS] + [os.path.join(REPO, "include", "lbm_hip.h")]
-    newest_dep = max(_mtime(d) for d in deps)
     jobs = []
     objs = []
     for src, flags in _units():
         s = os.path.join(CSRC, src)
         o = os.path.join(BUILD, src.rsplit(".", 1)[0] + ".o")
         objs.append(o)
+        newest_dep = max(_mtime(d) for d in _deps(src))
         if not force and _mtime(o) >= max(_mtime(s), newest_dep):
             continue
         jobs.append((src, [HIPCC, *COMMON, *flags, "-c", s, "-o", o]))

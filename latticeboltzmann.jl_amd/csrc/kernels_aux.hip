@@ -571,6 +571,136 @@ int dispatch_copy(const void *src, void *dst, size_t nbytes, hipStream_t s) {
     return hipGetLastError() == hipSuccess ? 0 : -4;
 }
 
+// ---------------------------------------------------------------------------
+// Analytic initial conditions (initial_conditions/analytical_equilibrium.jl,
+// analytical_offequilibrium.jl) per node, in the reference's FP64 order: the
+// Hermite-sum equilibrium w_i (rho + sum_n a_eq[n] : H_n(c_i) / (n! cs^n))
+// with the temperature terms of equilibrium_coefficient (hermite.jl:37-77,
+// Val{4} with its index pairs as written there), plus the off-equilibrium
+// part of the two off-equilibrium strategies.  cs^n comes from the host's
+// libm pow (Julia's Float64 ^ Int).  One thread per node; the tensors are
+// formed from their integer parts at run time (a one-off pass).
+// ---------------------------------------------------------------------------
+__device__ double herm_entry(int n, int cx, int cy, int k, double cs) {
+    const int xi[2] = {cx, cy};
+    const int a = k & 1, b = (k >> 1) & 1, g = (k >> 2) & 1, e = (k >> 3) & 1;
+    auto dl = [](int i, int j) { return i == j ? 1 : 0; };
+    if (n == 1) return (double)xi[k] * 1.0;
+    if (n == 2) return (double)(xi[b] * xi[a]) - cs * (double)dl(a, b);
+    if (n == 3) {
+        const int A = xi[g] * xi[b] * xi[a];
+        const int B = xi[a] * dl(b, g) + xi[b] * dl(a, g) + xi[g] * dl(a, b);
+        return (double)A - cs * (double)B;
+    }
+    const int A = xi[e] * xi[g] * xi[b] * xi[a];
+    const int B = xi[a] * xi[b] * dl(g, e) + xi[a] * xi[g] * dl(b, e) + xi[a] * xi[e] * dl(b, g) +
+                  xi[b] * xi[g] * dl(a, e) + xi[b] * xi[e] * dl(a, g) + xi[g] * xi[e] * dl(a, b);
+    const int C = dl(a, b) * dl(g, e) + dl(a, g) * dl(b, e) + dl(a, e) * dl(b, g);
+    return ((double)A - cs * (double)B) + (cs * cs) * (double)C;
+}
+
+__device__ double eq_coefficient_T(int n, int k, double rho, const double (&u)[2], double T,
+                                   double cs) {
+    if (n == 1) return rho * u[k];
+    const double ct = cs * (T - 1.0);
+    const int a = k & 1, b = (k >> 1) & 1, c = (k >> 2) & 1, d = (k >> 3) & 1;
+    auto dl = [](int i, int j) { return i == j ? 1.0 : 0.0; };
+    double v;
+    if (n == 2) {
+        v = u[a] * u[b];
+        if (a == b) v = v + ct;
+    } else if (n == 3) {
+        const double S = (u[a] * dl(b, c) + u[b] * dl(a, c)) + u[c] * dl(a, b);
+        v = ((u[a] * u[b]) * u[c]) + ct * S;
+    } else {
+        const double S1 = (((((u[a] * u[b]) * dl(c, d) + (u[a] * u[c]) * dl(b, d)) +
+                             (u[a] * u[d]) * dl(b, d)) +
+                            (u[b] * u[c]) * dl(a, d)) +
+                           (u[b] * u[d]) * dl(a, d)) +
+                          (u[c] * u[d]) * dl(a, b);
+        const int S0 = (int)(dl(a, b) * dl(c, d) + dl(a, c) * dl(b, d)) + (int)(dl(a, d) * dl(b, c));
+        v = ((((u[a] * u[b]) * u[c]) * u[d]) + ct * S1) + ((cs * cs) * ((T - 1.0) * (T - 1.0))) * (double)S0;
+    }
+    return rho * v;
+}
+
+template <int QID>
+__global__ void __launch_bounds__(256)
+k_init_analytical(double *__restrict__ f, const StepParams<double> p, const InitArgs a) {
+    constexpr int Q = QT<QID>::Q;
+    constexpr int N = QT<QID>::ORDER / 2;
+    const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= p.plane) return;
+    const double cs = p.cs_inv;
+    const double rho = a.rho[n];
+    const double u[2] = {a.ux[n], a.uy[n]};
+    const double T = a.T ? a.T[n] : 1.0;
+    double S[4] = {0.0, 0.0, 0.0, 0.0};
+    if (a.mode != 0)
+        for (int b = 0; b < 2; ++b)
+            for (int c = 0; c < 2; ++c)
+                S[c + 2 * b] = a.G[(c + 2 * b) * p.plane + n] + a.G[(b + 2 * c) * p.plane + n];
+    for (int i = 0; i < Q; ++i) {
+        const int cx = QT<QID>::cx(i), cy = QT<QID>::cy(i);
+        double sum = 0.0;
+        for (int m = 1; m <= N; ++m) {
+            double d = 0.0;
+            for (int k = 0; k < (1 << m); ++k) {
+                const double t = eq_coefficient_T(m, k, rho, u, T, cs) * herm_entry(m, cx, cy, k, cs);
+                d = (k == 0) ? t : d + t;
+            }
+            const double term = d / (a.fact[m] * a.csn[m]);
+            sum = (m == 1) ? term : sum + term;
+        }
+        double fi = p.w[i] * (rho + sum);
+        if (a.mode != 0) {
+            double d2 = herm_entry(2, cx, cy, 0, cs) * S[0];
+            for (int k = 1; k < 4; ++k) d2 = d2 + herm_entry(2, cx, cy, k, cs) * S[k];
+            if (a.mode == 1)
+                fi = fi - ((p.w[i] * (((p.s * (a.tau + 0.5)) * rho) * 1.0)) / 2.0) * d2;
+            else
+                fi = fi + ((((-a.factor) * p.w[i]) * 0.5) * ((a.tau + 0.5) * p.s)) * d2;
+        }
+        f[(long long)i * p.plane + n] = fi;
+    }
+}
+
+int dispatch_init_analytical(int qid, double *f, const StepParams<double> &p, const InitArgs &a,
+                             hipStream_t s) {
+    const unsigned blocks = (unsigned)((p.plane + 255) / 256);
+    switch (qid) {
+        case 4: k_init_analytical<4><<<blocks, 256, 0, s>>>(f, p, a); break;
+        case 5: k_init_analytical<5><<<blocks, 256, 0, s>>>(f, p, a); break;
+        case 9: k_init_analytical<9><<<blocks, 256, 0, s>>>(f, p, a); break;
+        case 13: k_init_analytical<13><<<blocks, 256, 0, s>>>(f, p, a); break;
+        case 17: k_init_analytical<17><<<blocks, 256, 0, s>>>(f, p, a); break;
+        case 21: k_init_analytical<21><<<blocks, 256, 0, s>>>(f, p, a); break;
+        case 37: k_init_analytical<37><<<blocks, 256, 0, s>>>(f, p, a); break;
+        default: return -1;
+    }
+    return hipGetLastError() == hipSuccess ? 0 : -4;
+}
+
+// Clock probe: lane 0 of each block writes (XCC id, shader-clock counter,
+// constant-rate counter).  Blocks are dealt round-robin over the XCDs, so
+// 64 blocks sample every XCD; the counters of one XCD are compared between two
+// probes.  s_getreg field: HW_REG_XCC_ID (20), bits [3:0].
+__global__ void __launch_bounds__(64) k_clock_probe(unsigned long long *out) {
+    if (threadIdx.x != 0) return;
+    const unsigned xcc = __builtin_amdgcn_s_getreg((3 << 11) | 20) & 15u;
+    const unsigned long long t = __builtin_amdgcn_s_memtime();
+    const unsigned long long r = __builtin_amdgcn_s_memrealtime();
+    out[3 * blockIdx.x] = xcc;
+    out[3 * blockIdx.x + 1] = t;
+    out[3 * blockIdx.x + 2] = r;
+}
+
+int dispatch_clock_probe(unsigned long long *out, int blocks, hipStream_t s) {
+    if (blocks < 1 || blocks > 1024) return -1;
+    k_clock_probe<<<blocks, 64, 0, s>>>(out);
+    return hipGetLastError() == hipSuccess ? 0 : -4;
+}
+
 template <int QID>
 static int track_q(int storage, const void *f, const StepParams<double> &p, const TrackArgs &a,
                    double *partials, int npart, hipStream_t s) {

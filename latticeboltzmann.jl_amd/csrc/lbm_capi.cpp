@@ -402,6 +402,24 @@ int lbm_synchronize(void) {
     return 0;
 }
 
+int lbm_clock_sample(uint64_t *out, double *tick_mhz) {
+    if (!out) return fail(-1, "lbm_clock_sample: null output");
+    if (int rc = ensure_stream()) return rc;
+    static unsigned long long *d = nullptr;
+    if (!d) HIP_TRY(hipMalloc(&d, 3 * 64 * sizeof(unsigned long long)));
+    if (lbm::dispatch_clock_probe(d, 64, g_stream)) return fail(-4, "clock probe launch failed");
+    HIP_TRY(hipMemcpyAsync(out, d, 3 * 64 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                           g_stream));
+    HIP_TRY(hipStreamSynchronize(g_stream));
+    if (tick_mhz) {
+        int dev = 0, khz = 0;
+        HIP_TRY(hipGetDevice(&dev));
+        HIP_TRY(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+        *tick_mhz = khz / 1000.0;
+    }
+    return 0;
+}
+
 int lbm_quadrature_info(int quadrature, int *Q, double *cx, double *cy, double *w, double *s,
                         int *order, int *opp) {
     HostQuad q;
@@ -540,6 +558,62 @@ int lbm_field_upload(lbm_field f, const double *host) {
 }
 
 int lbm_field_download(lbm_field f, double *host) { return transfer(f, host, false); }
+
+int lbm_initialize_analytical(lbm_field f, int strategy, const double *rho, const double *ux,
+                              const double *uy, const double *T, const double *grad, double tau,
+                              double factor) {
+    if (int rc = check_field(f, "lbm_initialize_analytical")) return rc;
+    if (strategy < LBM_INIT_EQUILIBRIUM || strategy > LBM_INIT_OFFEQ)
+        return fail(-1, "lbm_initialize_analytical: unknown strategy %d", strategy);
+    if (!rho || !ux || !uy) return fail(-1, "lbm_initialize_analytical: null density or velocity");
+    if (strategy != LBM_INIT_EQUILIBRIUM && !grad)
+        return fail(-1, "lbm_initialize_analytical: the off-equilibrium start needs the gradient");
+    const size_t plane = (size_t)f->nx * f->ny;
+    const int nin = 3 + (T ? 1 : 0) + (strategy != LBM_INIT_EQUILIBRIUM ? 4 : 0);
+    double *din = nullptr, *dout = nullptr;
+    HIP_TRY(hipMalloc(&din, nin * plane * sizeof(double)));
+    hipError_t e = hipSuccess;
+    int rc = 0;
+    lbm::InitArgs a{};
+    size_t off = 0;
+    auto put = [&](const double *h, int nplanes) -> const double * {
+        double *d = din + off * plane;
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(d, h, nplanes * plane * sizeof(double), hipMemcpyHostToDevice,
+                               g_stream);
+        off += (size_t)nplanes;
+        return d;
+    };
+    a.rho = put(rho, 1);
+    a.ux = put(ux, 1);
+    a.uy = put(uy, 1);
+    a.T = T ? put(T, 1) : nullptr;
+    a.G = strategy != LBM_INIT_EQUILIBRIUM ? put(grad, 4) : nullptr;
+    a.mode = strategy;
+    a.tau = tau;
+    a.factor = factor;
+    const lbm::StepParams<double> p = field_params(f);
+    // cs^n as Julia's Float64 ^ Int (libm pow), n! as Float64
+    for (int n = 1; n <= 4; ++n) {
+        a.csn[n] = std::pow(p.cs_inv, (double)n);
+        double fa = 1.0;
+        for (int k = 2; k <= n; ++k) fa *= k;
+        a.fact[n] = fa;
+    }
+    if (e == hipSuccess && f->storage != LBM_STORE_F64)
+        e = hipMalloc(&dout, (size_t)f->Q * plane * sizeof(double));
+    if (e == hipSuccess) {
+        double *target = f->storage == LBM_STORE_F64 ? static_cast<double *>(f->d) : dout;
+        rc = lbm::dispatch_init_analytical(f->qid, target, p, a, g_stream);
+        if (!rc && dout) rc = lbm::dispatch_upload(f->qid, f->storage, dout, f->d, p, 1, g_stream);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(g_stream);
+    (void)hipFree(din);
+    if (dout) (void)hipFree(dout);
+    if (e != hipSuccess) return fail(-3, "lbm_initialize_analytical: %s", hipGetErrorString(e));
+    if (rc) return fail(-4, "lbm_initialize_analytical: kernel failed (%d)", rc);
+    return 0;
+}
 
 int lbm_field_copy(lbm_field dst, lbm_field src) {
     if (int rc = check_field(dst, "lbm_field_copy")) return rc;
@@ -745,7 +819,7 @@ int lbm_stream_bc(lbm_model m, lbm_field f, lbm_field f_new) {
 }
 
 static int simulate_impl(lbm_model m, lbm_field S, lbm_field C, int64_t nsteps, double *ms,
-                         int64_t *nlaunch) {
+                         int64_t *nlaunch, double *per_launch = nullptr) {
     if (int rc = check_pair(m, S, C, "lbm_simulate", true, true)) return rc;
     if (nsteps < 0) return fail(-1, "negative step count");
     if (ms) *ms = 0.0;
@@ -769,11 +843,34 @@ static int simulate_impl(lbm_model m, lbm_field S, lbm_field C, int64_t nsteps, 
         HIP_TRY(hipEventCreate(&e1));
         HIP_TRY(hipEventRecord(e0, g_stream));
     }
+    // per-launch events (measurement runs only): one before every fused launch
+    std::vector<hipEvent_t> ev;
+    if (per_launch) {
+        ev.assign((size_t)(nsteps > 1 ? nsteps : 1), nullptr);
+        for (auto &e : ev) HIP_TRY(hipEventCreate(&e));
+    }
     // steps 2..n: C_k = collide(BC(stream(C_{k-1}))) in one pull pass each
     lbm_field cur = C, other = S;
     for (int64_t k = 1; k < nsteps; ++k) {
+        if (per_launch) HIP_TRY(hipEventRecord(ev[(size_t)k - 1], g_stream));
         if (int rc = run(lbm::Op::Fused, m, cur, other, true)) return rc;
         std::swap(cur, other);
+    }
+    if (per_launch) {
+        HIP_TRY(hipEventRecord(ev.back(), g_stream));
+        HIP_TRY(hipEventSynchronize(ev.back()));
+        for (int64_t k = 1; k + 1 < nsteps; ++k) {
+            float t = 0.0f;
+            HIP_TRY(hipEventElapsedTime(&t, ev[(size_t)k - 1], ev[(size_t)k]));
+            per_launch[k - 1] = (double)t;
+        }
+        if (nsteps > 1) {
+            float t = 0.0f;
+            HIP_TRY(hipEventElapsedTime(&t, ev[(size_t)nsteps - 2], ev.back()));
+            per_launch[nsteps - 2] = (double)t;
+        }
+        for (auto &e : ev) (void)hipEventDestroy(e);
+        if (nlaunch) *nlaunch = nsteps - 1;
     }
     if (ms) {
         HIP_TRY(hipEventRecord(e1, g_stream));
@@ -822,6 +919,8 @@ int lbm_autotune(lbm_model m, lbm_field f_stream, int trial_steps, int *block_x,
     if (!m) return fail(-1, "null model");
     if (int rc = check_field(f_stream, "lbm_autotune")) return rc;
     if (f_stream->qid != m->qid) return fail(-1, "lbm_autotune: quadrature mismatch");
+    if (m->coll == LBM_MEI)
+        return fail(-2, "lbm_autotune: the iterative initialization model has no fused step");
     if (trial_steps < 1) trial_steps = 1;
     // scratch copies: tuning must not advance the caller's state
     lbm_field A = nullptr, B = nullptr;
@@ -829,6 +928,9 @@ int lbm_autotune(lbm_model m, lbm_field f_stream, int trial_steps, int *block_x,
                               f_stream->rho0);
     if (!rc) rc = lbm_field_create(&B, f_stream->qid, f_stream->nx, f_stream->ny,
                                    f_stream->storage, f_stream->rho0);
+    // the model/field checks of lbm_simulate (force field shape, BC ranges,
+    // MRT storage) before any kernel reads the model's buffers
+    if (!rc) rc = check_pair(m, A, B, "lbm_autotune", true, true);
     if (!rc) rc = lbm_field_copy(A, f_stream);
     const int save_b = m->launch_block, save_r = m->launch_rows, save_v = m->launch_vec,
               save_n = m->launch_nt;
@@ -895,6 +997,12 @@ int lbm_autotune(lbm_model m, lbm_field f_stream, int trial_steps, int *block_x,
     if (nontemporal) *nontemporal = bn;
     if (us_per_step) *us_per_step = best;
     return 0;
+}
+
+int lbm_simulate_launch_times(lbm_model m, lbm_field f_stream, lbm_field f_collision,
+                              int64_t nsteps, double *launch_ms, int64_t *nlaunch) {
+    if (!launch_ms) return fail(-1, "lbm_simulate_launch_times: null output");
+    return simulate_impl(m, f_stream, f_collision, nsteps, nullptr, nlaunch, launch_ms);
 }
 
 int lbm_simulate(lbm_model m, lbm_field f_stream, lbm_field f_collision, int64_t nsteps) {
@@ -1247,9 +1355,20 @@ namespace {
 
 int batch_elem(const lbm_batch_s *b) { return storage_bytes(b->storage); }
 
-// the tile table for tiles of `block` nodes
+void batch_drop_graph(lbm_batch b) {
+    if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->gexec = nullptr;
+    b->graph = nullptr;
+    b->g_chunk = 0;
+    b->g_parity = -1;
+}
+
+// the tile table for tiles of `block` nodes.  A captured graph holds the old
+// table's address in its kernel nodes, so it is dropped with the table.
 int batch_tiles(lbm_batch b, int block) {
     if (b->tile_block == block && b->dTiles) return 0;
+    batch_drop_graph(b);
     std::vector<lbm::BatchTile> tiles;
     for (int k = 0; k < b->n; ++k)
         for (long long n0 = 0; n0 < b->lats[k].plane; n0 += block)
@@ -1303,15 +1422,6 @@ int batch_launch(lbm_batch b, int op, int parity) {
                                    grid, b->vec, lbm::to_float_params(p), g_stream);
     if (rc) return fail(-4, "batch kernel launch failed (%d)", rc);
     return 0;
-}
-
-void batch_drop_graph(lbm_batch b) {
-    if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
-    if (b->graph) (void)hipGraphDestroy(b->graph);
-    b->gexec = nullptr;
-    b->graph = nullptr;
-    b->g_chunk = 0;
-    b->g_parity = -1;
 }
 
 // capture `chunk` fused steps starting at `parity` (chunk even: parity returns)
@@ -1932,6 +2042,190 @@ int lbm_adjoint_gradient(lbm_adjoint a, const double *u0, const double *u_target
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
     }
+    return 0;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// snapshots: TakeSnapshots (take_snapshots.jl:3-29) without stalling the step
+// ---------------------------------------------------------------------------
+// A snapshot is copy(f_in) in the reference's (nx, ny, Q) Float64 layout.
+// lbm_snapshot_take decodes / copies the field into one of a few staging
+// buffers in HBM on the library stream (a pass over the field), then a second
+// stream copies that buffer into pinned host memory and, when a directory was
+// given, a host callback on that stream writes the .npy file.  The library
+// stream runs on meanwhile; it waits only to reuse a staging buffer whose
+// copy to the host has not finished.  The staging buffers bound the HBM held
+// (slots x nx x ny x Q x 8 bytes); the host keeps every snapshot, as the
+// reference's vector of arrays does.
+struct SnapshotHost {
+    int64_t t = 0;
+    double *host = nullptr;  // pinned
+    std::string path;        // empty: not written
+    size_t bytes = 0;
+    int nx = 0, ny = 0, Q = 0;
+    int write_rc = 0;
+};
+
+struct lbm_snapshots_s {
+    int qid = 0, nx = 0, ny = 0, Q = 0, slots = 0;
+    size_t bytes = 0;
+    std::string dir;
+    hipStream_t copy = nullptr;
+    std::vector<double *> staging;
+    std::vector<hipEvent_t> staged, freed;
+    std::vector<bool> used;
+    std::vector<SnapshotHost *> shots;
+};
+
+namespace {
+
+// .npy version 1.0: little-endian Float64, Fortran order, shape (nx, ny, Q)
+int write_npy(const SnapshotHost &s) {
+    FILE *fp = std::fopen(s.path.c_str(), "wb");
+    if (!fp) return -1;
+    char dict[160];
+    std::snprintf(dict, sizeof dict,
+                  "{'descr': '<f8', 'fortran_order': True, 'shape': (%d, %d, %d), }", s.nx, s.ny,
+                  s.Q);
+    std::string header(dict);
+    // magic (6) + version (2) + header length (2) + header, padded to 64 bytes
+    size_t total = 10 + header.size() + 1;
+    header.append((64 - total % 64) % 64, ' ');
+    header.push_back('\n');
+    const unsigned short hl = (unsigned short)header.size();
+    const unsigned char pre[10] = {0x93, 'N', 'U', 'M', 'P', 'Y', 1, 0,
+                                   (unsigned char)(hl & 0xff), (unsigned char)(hl >> 8)};
+    bool ok = std::fwrite(pre, 1, 10, fp) == 10 &&
+              std::fwrite(header.data(), 1, header.size(), fp) == header.size() &&
+              std::fwrite(s.host, 1, s.bytes, fp) == s.bytes;
+    ok = (std::fclose(fp) == 0) && ok;
+    return ok ? 0 : -1;
+}
+
+void write_npy_cb(void *arg) {
+    SnapshotHost *s = static_cast<SnapshotHost *>(arg);
+    s->write_rc = write_npy(*s);
+}
+
+}  // namespace
+
+extern "C" {
+
+int lbm_snapshots_create(lbm_snapshots *out, int quadrature, int nx, int ny, int device_slots,
+                         const char *directory) {
+    if (!out) return fail(-1, "lbm_snapshots_create: null output handle");
+    *out = nullptr;
+    HostQuad q;
+    if (!host_quad(quadrature, q)) return fail(-1, "unknown quadrature D2Q%d", quadrature);
+    if (nx < 1 || ny < 1) return fail(-1, "lattice size must be positive");
+    if (device_slots < 1 || device_slots > 64) return fail(-1, "device_slots must be 1..64");
+    if (int rc = ensure_stream()) return rc;
+    auto *s = new lbm_snapshots_s;
+    s->qid = quadrature;
+    s->nx = nx;
+    s->ny = ny;
+    s->Q = q.Q;
+    s->slots = device_slots;
+    s->bytes = (size_t)nx * ny * q.Q * sizeof(double);
+    if (directory) s->dir = directory;
+    hipError_t e = hipStreamCreate(&s->copy);
+    for (int k = 0; k < device_slots && e == hipSuccess; ++k) {
+        double *d = nullptr;
+        e = hipMalloc(&d, s->bytes);
+        if (e == hipSuccess) s->staging.push_back(d);
+        hipEvent_t a = nullptr, b = nullptr;
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&a, hipEventDisableTiming);
+        if (e == hipSuccess) s->staged.push_back(a);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&b, hipEventDisableTiming);
+        if (e == hipSuccess) s->freed.push_back(b);
+        s->used.push_back(false);
+    }
+    if (e != hipSuccess) {
+        lbm_snapshots_destroy(s);
+        return fail(-3, "lbm_snapshots_create: %s", hipGetErrorString(e));
+    }
+    *out = s;
+    return 0;
+}
+
+int lbm_snapshots_destroy(lbm_snapshots s) {
+    if (!s) return 0;
+    if (s->copy) (void)hipStreamSynchronize(s->copy);
+    if (g_stream) (void)hipStreamSynchronize(g_stream);
+    for (double *d : s->staging) (void)hipFree(d);
+    for (hipEvent_t e : s->staged) (void)hipEventDestroy(e);
+    for (hipEvent_t e : s->freed) (void)hipEventDestroy(e);
+    for (SnapshotHost *h : s->shots) {
+        if (h->host) (void)hipHostFree(h->host);
+        delete h;
+    }
+    if (s->copy) (void)hipStreamDestroy(s->copy);
+    delete s;
+    return 0;
+}
+
+int lbm_snapshot_take(lbm_snapshots s, lbm_field f, int64_t t) {
+    if (!s) return fail(-1, "null snapshots handle");
+    if (int rc = check_field(f, "lbm_snapshot_take")) return rc;
+    if (f->qid != s->qid || f->nx != s->nx || f->ny != s->ny)
+        return fail(-1, "lbm_snapshot_take: field is D2Q%d %dx%d, the snapshots D2Q%d %dx%d",
+                    f->qid, f->nx, f->ny, s->qid, s->nx, s->ny);
+    const int j = (int)(s->shots.size() % (size_t)s->slots);
+    auto *h = new SnapshotHost;
+    h->t = t;
+    h->bytes = s->bytes;
+    h->nx = s->nx;
+    h->ny = s->ny;
+    h->Q = s->Q;
+    hipError_t e = hipHostMalloc(reinterpret_cast<void **>(&h->host), s->bytes);
+    if (e != hipSuccess) {
+        delete h;
+        return fail(-3, "lbm_snapshot_take: pinned host buffer: %s", hipGetErrorString(e));
+    }
+    if (!s->dir.empty()) {
+        char name[64];
+        std::snprintf(name, sizeof name, "/snapshot_%010lld.npy", (long long)t);
+        h->path = s->dir + name;
+    }
+    s->shots.push_back(h);
+    // the staging buffer is reused only after its previous copy to the host
+    if (s->used[j]) HIP_TRY(hipStreamWaitEvent(g_stream, s->freed[j], 0));
+    s->used[j] = true;
+    if (f->storage == LBM_STORE_F64) {
+        if (int rc = lbm::dispatch_copy(f->d, s->staging[j], s->bytes, g_stream))
+            return fail(-4, "snapshot copy kernel failed (%d)", rc);
+    } else {
+        const lbm::StepParams<double> p = field_params(f);
+        if (int rc = lbm::dispatch_upload(f->qid, f->storage, s->staging[j], f->d, p, 0, g_stream))
+            return fail(-4, "snapshot decode kernel failed (%d)", rc);
+    }
+    HIP_TRY(hipEventRecord(s->staged[j], g_stream));
+    HIP_TRY(hipStreamWaitEvent(s->copy, s->staged[j], 0));
+    HIP_TRY(hipMemcpyAsync(h->host, s->staging[j], s->bytes, hipMemcpyDeviceToHost, s->copy));
+    if (!h->path.empty()) HIP_TRY(hipLaunchHostFunc(s->copy, write_npy_cb, h));
+    HIP_TRY(hipEventRecord(s->freed[j], s->copy));
+    return 0;
+}
+
+int lbm_snapshots_wait(lbm_snapshots s, int *n) {
+    if (!s) return fail(-1, "null snapshots handle");
+    HIP_TRY(hipStreamSynchronize(s->copy));
+    for (const SnapshotHost *h : s->shots)
+        if (h->write_rc) return fail(-3, "writing %s failed", h->path.c_str());
+    if (n) *n = (int)s->shots.size();
+    return 0;
+}
+
+int lbm_snapshot_get(lbm_snapshots s, int k, int64_t *t, double *host) {
+    if (!s) return fail(-1, "null snapshots handle");
+    if (k < 0 || (size_t)k >= s->shots.size())
+        return fail(-1, "snapshot %d of %d", k, (int)s->shots.size());
+    HIP_TRY(hipStreamSynchronize(s->copy));
+    const SnapshotHost *h = s->shots[(size_t)k];
+    if (t) *t = h->t;
+    if (host) std::memcpy(host, h->host, h->bytes);
     return 0;
 }
 

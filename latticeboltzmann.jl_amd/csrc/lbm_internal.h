@@ -194,5 +194,20 @@ int dispatch_gather_node(int storage, const void *f, const StepParams<double> &p
                          int Q, double *out, hipStream_t s);
 // f[plane * i + n] = w_i for every node (plain FP64 storage)
 int dispatch_fill_weights(double *f, const StepParams<double> &p, int Q, hipStream_t s);
+// analytic initial conditions: device planes rho, ux, uy, T (null: 1) and
+// G (4 planes, G[a + 2b]; null for mode 0); mode 0 equilibrium, 1
+// off-equilibrium of the Taylor-Green vortex, 2 off-equilibrium (generic)
+struct InitArgs {
+    const double *rho, *ux, *uy, *T, *G;
+    int mode;
+    double tau, factor;
+    double csn[5];   // cs^n, n = 1..4
+    double fact[5];  // n!
+};
+// writes the FP64 populations (nx, ny, Q) of the lattice of p
+int dispatch_init_analytical(int qid, double *f, const StepParams<double> &p, const InitArgs &a,
+                             hipStream_t s);
+// (XCC id, shader-clock counter, constant-rate counter) per block, 3 x blocks values
+int dispatch_clock_probe(unsigned long long *out, int blocks, hipStream_t s);
 
 }  // namespace lbm

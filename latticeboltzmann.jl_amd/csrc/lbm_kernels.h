@@ -505,6 +505,78 @@ __device__ __forceinline__ void collide_seq(T (&f)[QT<QID>::Q], const StepParams
 // above); the stored part adds the difference between the exact equilibrium
 // and that of the rounded record.  Returned multiplied by the storage scale,
 // with the rounded record of the node.
+//
+// For the first- and second-order sets it runs one opposite pair (i, j =
+// opp(i), c_j = -c_i) at a time.  A node's shifted equilibrium splits into an
+// even and an odd part: feq_i = w (Ev + Od), feq_j = w (Ev - Od) with the
+// record's Ev = a0 + a3 (c.a)^2, Od = c.a.  The stored part
+//   nq_i = k_a n_i + k_b n_j + (feq_i(exact) - feq_i(record)),
+//   n_i = f_i - w (Ev + Od),  n_j = f_j - w (Ev - Od),
+//   feq_i(exact) - feq_i(record) = w ((Ev - Ev_q) + (Od - Od_q))
+// (subscript q: the rounded record; SRT: k_b = 0).  A pair costs two even and
+// two odd parts instead of four full equilibria.  The non-equilibrium parts
+// stay differences of nearly equal values (exact in FP32), as in the
+// per-population form: relaxing the full populations instead (k_a f_i +
+// k_b f_j + ...) rounds at their size and doubled the FP16-NEQ velocity
+// error of the 4096^2 vortex after 5000 steps (5e-5 -> 1.4e-4).
+template <int QID, int COLL, class ST, int... Is>
+__device__ __forceinline__ void collide_neq_pairs(const float (&f)[QT<QID>::Q],
+                                                  const StepParams<float> &p, int x, int y,
+                                                  Rec16 &rec, float (&nq)[QT<QID>::Q],
+                                                  std::integer_sequence<int, Is...>) {
+    constexpr int FEQ = QT<QID>::FEQ;
+    static_assert(FEQ <= 2, "the pair form uses the record form of the equilibrium");
+    float drho, ux, uy;
+    node_moments_dev<QID, float>(f, p, drho, ux, uy);
+    if (p.force_kind != 0) {
+        float Fx, Fy;
+        node_force(p, x, y, Fx, Fy);
+        ux = ux + p.tau_force * Fx;
+        uy = uy + p.tau_force * Fy;
+    }
+    const Mom4 mt = make_rec<QID>(drho, ux, uy, p);
+    rec = ST::pack(mt);
+    const Mom4 mq = ST::unpack(rec);
+    // even and odd parts of the node's (exact or rounded) equilibrium along c_I
+    auto parts = [&](auto Ic, const Mom4 &m, float &ev, float &od) {
+        constexpr int I = decltype(Ic)::value;
+        od = udot<QID, I>(m.a1, m.a2);
+        ev = FEQ == 2 ? m.a0 + m.a3 * (od * od) : m.a0;
+    };
+    float ka, kb;
+    if constexpr (COLL == 0) {
+        ka = p.omo * p.dscale;
+        kb = 0.0f;
+    } else {
+        ka = (1.0f - 0.5f * (p.oms + p.oma)) * p.dscale;
+        kb = (0.5f * (p.oma - p.oms)) * p.dscale;
+    }
+    auto pair = [&](auto Ic) {
+        constexpr int i = decltype(Ic)::value;
+        constexpr int j = QT<QID>::opp(i);
+        if constexpr (i <= j) {
+            const float w = p.w[i], wd = w * p.dscale;
+            float ev, od, evq, odq;
+            parts(Ic, mt, ev, od);
+            parts(Ic, mq, evq, odq);
+            const float dE = ev - evq;
+            if constexpr (i == j) {
+                nq[i] = (ka + kb) * (f[i] - w * ev) + wd * dE;
+            } else {
+                const float dO = od - odq;
+                const float ni = f[i] - w * (ev + od), nj = f[j] - w * (ev - od);
+                nq[i] = ka * ni + (kb * nj + wd * (dE + dO));
+                nq[j] = ka * nj + (kb * ni + wd * (dE - dO));
+            }
+        }
+    };
+    (pair(std::integral_constant<int, Is>{}), ...);
+}
+
+// The per-population form of the same collision, for the third- and
+// fourth-order sets: there the opposite-pair split of the Horner polynomial
+// costs registers and scheduling (D2Q37 FP16-NEQ 8192 x 2048: 1.09 ms with
+// the pair form, 0.89 ms with this one).
 template <int QID, int COLL, class ST, int... Is>
 __device__ __forceinline__ void collide_neq_seq(const float (&f)[QT<QID>::Q],
                                                 const StepParams<float> &p, int x, int y,
@@ -541,6 +613,16 @@ __device__ __forceinline__ void collide_neq_seq(const float (&f)[QT<QID>::Q],
         const float kb = (0.5f * (p.oma - p.oms)) * p.dscale;
         ((nq[Is] = (ka * n[Is] + kb * n[QT<QID>::opp(Is)]) + d[Is]), ...);
     }
+}
+
+// the NEQ collision of a node: the opposite-pair form for the first- and
+// second-order sets, the per-population form for the third- and fourth-order
+template <int QID, int COLL, class ST, int... Is>
+__device__ __forceinline__ void collide_neq(const float (&f)[QT<QID>::Q], const StepParams<float> &p,
+                                            int x, int y, Rec16 &rec, float (&nq)[QT<QID>::Q],
+                                            std::integer_sequence<int, Is...> seq) {
+    if constexpr (QT<QID>::FEQ <= 2) collide_neq_pairs<QID, COLL, ST>(f, p, x, y, rec, nq, seq);
+    else collide_neq_seq<QID, COLL, ST>(f, p, x, y, rec, nq, seq);
 }
 
 template <int QID, int COLL, bool SHIFT, class T>
@@ -972,6 +1054,68 @@ __device__ __forceinline__ U *at32(U *base, int idx) {
                                  static_cast<unsigned>(idx) * static_cast<unsigned>(sizeof(U)));
 }
 
+// Wave-uniform row addressing inside a loop.  at_s(base, idx) gives
+// base + idx with the base made an opaque, wave-uniform value (see
+// opaque_uniform) and the 32-bit lane offset formed at the access.  Without
+// this the compiler hoists the zero-extended lane offset out of the loop,
+// shares one 64-bit VGPR address between planes and adds each plane's
+// offset with 64-bit VALU arithmetic, instead of using the scalar-base
+// ("saddr") form of the global load or store.
+template <class U>
+using gptr = __attribute__((address_space(1))) U *;
+
+// A wave-uniform 64-bit integer the compiler can neither hoist out of a loop
+// nor treat as divergent: an asm statement hides its value (its result counts
+// as divergent), readfirstlane restores the uniformity (a copy on an SGPR).
+__device__ __forceinline__ long long opaque_uniform(long long v) {
+    asm volatile("" : "+s"(v));
+    const unsigned long long u = static_cast<unsigned long long>(v);
+    const unsigned lo = __builtin_amdgcn_readfirstlane(static_cast<unsigned>(u));
+    const unsigned hi = __builtin_amdgcn_readfirstlane(static_cast<unsigned>(u >> 32));
+    return static_cast<long long>((static_cast<unsigned long long>(hi) << 32) | lo);
+}
+
+__device__ __forceinline__ long long first_lane(long long v) {
+    const unsigned long long u = static_cast<unsigned long long>(v);
+    const unsigned lo = __builtin_amdgcn_readfirstlane(static_cast<unsigned>(u));
+    const unsigned hi = __builtin_amdgcn_readfirstlane(static_cast<unsigned>(u >> 32));
+    return static_cast<long long>((static_cast<unsigned long long>(hi) << 32) | lo);
+}
+
+// base + a byte offset: base wave-uniform (SGPR pair), off a lane's offset
+template <class U>
+__device__ __forceinline__ gptr<U> at_b(U *base, unsigned off) {
+    const long long b = first_lane(reinterpret_cast<long long>(base));
+    return reinterpret_cast<gptr<U>>(b + off);
+}
+// idx * size as a byte offset the compiler cannot hoist: formed once per loop
+// iteration and shared by every access of that iteration (each stencil
+// shift goes into the uniform base instead)
+__device__ __forceinline__ unsigned lane_bytes(int idx, unsigned size) {
+    unsigned off = static_cast<unsigned>(idx) * size;
+    asm volatile("" : "+v"(off));
+    return off;
+}
+
+// a load through a global pointer (copied as raw bits: class types cannot
+// bind a reference in the global address space)
+template <class U>
+__device__ __forceinline__ typename std::remove_const<U>::type ld_g(gptr<U> ptr) {
+    using V = typename std::remove_const<U>::type;
+    using R = typename RawOf<sizeof(V)>::type;
+    const R r = *reinterpret_cast<gptr<const R>>(ptr);
+    V v;
+    __builtin_memcpy(&v, &r, sizeof(V));
+    return v;
+}
+
+template <class U>
+__device__ __forceinline__ gptr<U> at_s(U *base, int idx) {
+    const long long b = first_lane(reinterpret_cast<long long>(base));
+    const unsigned off = static_cast<unsigned>(idx) * static_cast<unsigned>(sizeof(U));
+    return reinterpret_cast<gptr<U>>(b + off);
+}
+
 template <int QID, class ST, int VEC, int... Is>
 __device__ __forceinline__ void pull_interior_row(const typename ST::store_t *__restrict__ C,
                                                   const StepParams<typename ST::T> &p, int yw,
@@ -1009,6 +1153,14 @@ __device__ __forceinline__ void put_pack(S *dst, const Pack<S, VEC> &pk) {
     } else {
         __builtin_memcpy(dst, &pk, sizeof(pk));
     }
+}
+template <bool NT, class S, int VEC>
+__device__ __forceinline__ void put_pack(gptr<S> dst, const Pack<S, VEC> &pk) {
+    using R = typename RawOf<sizeof(pk)>::type;
+    R r;
+    __builtin_memcpy(&r, &pk, sizeof(pk));
+    if constexpr (NT) __builtin_nontemporal_store(r, reinterpret_cast<gptr<R>>(dst));
+    else *reinterpret_cast<gptr<R>>(dst) = r;
 }
 
 // Stores of VEC consecutive nodes starting at (x0, y).  ROW: wave-uniform row
@@ -1151,7 +1303,7 @@ k_fused_v(const typename ST::store_t *__restrict__ C, typename ST::store_t *__re
         float nq[VEC][QT<QID>::Q];
 #pragma unroll
         for (int v = 0; v < VEC; ++v)
-            collide_neq_seq<QID, COLL, ST>(f[v], p, x0 + v, y, rec[v], nq[v], LBM_SEQ);
+            collide_neq<QID, COLL, ST>(f[v], p, x0 + v, y, rec[v], nq[v], LBM_SEQ);
         if (row) store_vec<QID, ST, VEC, NT, true>(Cn, p, y, x0, f, LBM_SEQ, rec, nq);
         else store_vec<QID, ST, VEC, NT, false>(Cn, p, y, x0, f, LBM_SEQ, rec, nq);
     } else if constexpr (VEC == 1 && COLL != 2) {
@@ -1247,7 +1399,7 @@ __device__ __forceinline__ void neq_edge_update(const typename ST::store_t *__re
     pull_seq<QID, ST>(C, p, x, y, f, LBM_SEQ);
     Rec16 rec;
     float nq[Q];
-    collide_neq_seq<QID, COLL, ST>(f, p, x, y, rec, nq, LBM_SEQ);
+    collide_neq<QID, COLL, ST>(f, p, x, y, rec, nq, LBM_SEQ);
     const long long n = (long long)x + (long long)p.nx * y;
     ST::recs(Cn)[n] = rec;
     auto st = [&](auto Ic) {
@@ -1308,10 +1460,10 @@ k_fused_neq_rows(const typename ST::store_t *__restrict__ C,
     auto load_row = [&](int yy) {
         const Rec16 *r = row_of(yy);
         Rec16 *w = win[yy & (SL - 1)];
-        if (active) w[M + tid] = *at32(r, x);
+        if (active) w[M + tid] = ld_g(at_s(r, x));
         if (tid < M) {
-            w[tid] = *at32(r, xlw);
-            w[M + last + 1 + tid] = *at32(r, xrw);
+            w[tid] = ld_g(at_s(r, xlw));
+            w[M + last + 1 + tid] = ld_g(at_s(r, xrw));
         }
     };
 #pragma unroll
@@ -1320,37 +1472,59 @@ k_fused_neq_rows(const typename ST::store_t *__restrict__ C,
     // Software pipeline (one-speed stencils): the record of row y + 1 + M and
     // the populations of row y + 1 are loaded into registers while row y is
     // collided, and published / used one iteration later, so neither load's
-    // latency sits between a row's barrier and its arithmetic.  D2Q9
-    // FP16-NEQ 4096^2: 194 us against 197 us; on the wide stencils the extra
-    // Q registers cost more occupancy than the overlap gains (D2Q17 385 ->
-    // 423 us), so they load in the row itself.
+    // latency sits between a row's barrier and its arithmetic.  Two register
+    // sets alternate between even and odd rows (no copies).  D2Q9 FP16-NEQ
+    // 4096^2: 194 us against 197 us; on the wide stencils the extra Q
+    // registers cost more occupancy than the overlap gains (D2Q17 385 -> 423
+    // us), so they load in the row itself.
+    //
+    // Addresses (up to nine planes): every plane row's base is formed in
+    // scalar registers (the stencil shift included) from an opaque, per-row
+    // uniform plane size and row index, and every access adds one per-row lane
+    // byte offset, so the loads and stores take the scalar-base form with no
+    // 64-bit VALU address arithmetic (D2Q9 row loop: 316 -> 238 VALU
+    // instructions per node).  With 13 to 37 planes the 2Q live bases
+    // overflow the scalar registers (D2Q37: 423 spilled SGPRs, more VALU
+    // than before), so those keep 64-bit VGPR addresses.
     constexpr bool kPipe = M == 1;
+    constexpr bool kScalarBase = Q <= 9;
+    auto pop_in = [&](long long pl, int yy, unsigned o2, auto Ic) -> S {
+        constexpr int i = decltype(Ic)::value;
+        constexpr int cx = QT<QID>::cx(i), cy = QT<QID>::cy(i);
+        const S *row = ST::pops(C, pl) + pl * i + (long long)p.nx * (yy - cy);
+        if constexpr (kScalarBase) return ld_g(at_b(row - cx, o2));
+        else return *at32(row, x - cx);
+    };
     Rec16 rc{}, rl{}, rr{};
-    S h[kPipe ? Q : 1];
-    auto fetch = [&](int yy, long long pl) {
+    S hA[kPipe ? Q : 1], hB[kPipe ? Q : 1];
+    auto fetch = [&](int yy, long long pl, unsigned o2, unsigned o8, S (&h)[kPipe ? Q : 1]) {
         const Rec16 *r = row_of(yy + M);
-        if (active) rc = *at32(r, x);
+        if (active) {
+            if constexpr (kScalarBase) rc = ld_g(at_b(r, o8));
+            else rc = *at32(r, x);
+        }
         if (tid < M) {
-            rl = *at32(r, xlw);
-            rr = *at32(r, xrw);
+            rl = ld_g(at_s(r, xlw));
+            rr = ld_g(at_s(r, xrw));
         }
         if constexpr (kPipe) {
             if (xin && yy >= M && yy < p.ny - M) {
                 auto one = [&](auto Ic) {
                     constexpr int i = decltype(Ic)::value;
-                    constexpr int cx = QT<QID>::cx(i), cy = QT<QID>::cy(i);
-                    h[i] = *at32(ST::pops(C, pl) + pl * i + (long long)p.nx * (yy - cy), x - cx);
+                    h[i] = pop_in(pl, yy, o2, Ic);
                 };
                 adj_each_node<QID>(one);
             }
         }
     };
-    if constexpr (kPipe) fetch(y0, p.plane);
-#pragma unroll 1
-    for (int r = 0; r < R; ++r) {
-        const int y = y0 + r;
-        if (y >= p.ny) break;
-        S hc[kPipe ? Q : 1];
+    // one row; returns false past the last lattice row
+    auto row = [&](int r, S (&hc)[kPipe ? Q : 1], S (&hn)[kPipe ? Q : 1]) -> bool {
+        // scalar bases: an opaque uniform row, so no loop-carried (and
+        // VGPR-held) row offsets
+        const int y = kScalarBase ? __builtin_amdgcn_readfirstlane(y0 + r) : y0 + r;
+        if (y >= p.ny) return false;
+        const unsigned o2 = kScalarBase ? lane_bytes(x, sizeof(S)) : 0u;
+        const unsigned o8 = kScalarBase ? lane_bytes(x, sizeof(Rec16)) : 0u;
         if constexpr (kPipe) {
             Rec16 *w = win[(y + M) & (SL - 1)];
             if (active) w[M + tid] = rc;
@@ -1358,19 +1532,22 @@ k_fused_neq_rows(const typename ST::store_t *__restrict__ C,
                 w[tid] = rl;
                 w[M + last + 1 + tid] = rr;
             }
-#pragma unroll
-            for (int i = 0; i < Q; ++i) hc[i] = h[i];
         } else {
             load_row(y + M);
         }
         __syncthreads();
         // the plane size made opaque per row: the Q plane-row addresses are
-        // formed in scalar registers when used instead of being kept live
-        // across the loop (2Q registers for the wide stencils)
-        long long pl = p.plane;
-        asm volatile("" : "+s"(pl));
+        // formed when used instead of being kept live across the loop (2Q
+        // registers for the wide stencils)
+        long long pl;
+        if constexpr (kScalarBase) {
+            pl = opaque_uniform(p.plane);
+        } else {
+            pl = p.plane;
+            asm volatile("" : "+s"(pl));
+        }
         if constexpr (kPipe) {
-            if (r + 1 < R && y + 1 < p.ny) fetch(y + 1, pl);
+            if (r + 1 < R && y + 1 < p.ny) fetch(y + 1, pl, o2, o8, hn);
         }
         if (xin && y >= M && y < p.ny - M) {
             float f[Q];
@@ -1379,25 +1556,48 @@ k_fused_neq_rows(const typename ST::store_t *__restrict__ C,
                 constexpr int cx = QT<QID>::cx(i), cy = QT<QID>::cy(i);
                 S hv;
                 if constexpr (kPipe) hv = hc[i];
-                else hv = *at32(ST::pops(C, pl) + pl * i + (long long)p.nx * (y - cy), x - cx);
+                else hv = pop_in(pl, y, o2, Ic);
                 f[i] = geq_rec<QID, i>(ST::unpack(win[(y - cy) & (SL - 1)][M + tid - cx]), p) +
                        ST::to_f(hv, p);
             };
             adj_each_node<QID>(one);
             Rec16 rec;
             float nq[Q];
-            collide_neq_seq<QID, COLL, ST>(f, p, x, y, rec, nq, LBM_SEQ);
+            collide_neq<QID, COLL, ST>(f, p, x, y, rec, nq, LBM_SEQ);
             Pack<Rec16, 1> pm;
             pm.v[0] = rec;
-            put_pack<NT>(at32(ST::recs(Cn) + (long long)p.nx * y, x), pm);
+            Rec16 *rrow = ST::recs(Cn) + (long long)p.nx * y;
+            if constexpr (kScalarBase) put_pack<NT>(at_b(rrow, o8), pm);
+            else put_pack<NT>(at32(rrow, x), pm);
             auto st = [&](auto Ic) {
                 constexpr int i = decltype(Ic)::value;
                 Pack<S, 1> pk;
                 pk.v[0] = ST::raw(nq[i]);
-                put_pack<NT>(at32(ST::pops(Cn, pl) + pl * i + (long long)p.nx * y, x), pk);
+                S *row = ST::pops(Cn, pl) + pl * i + (long long)p.nx * y;
+                if constexpr (kScalarBase) put_pack<NT>(at_b(row, o2), pk);
+                else put_pack<NT>(at32(row, x), pk);
             };
             adj_each_node<QID>(st);
         }
+        return true;
+    };
+    if constexpr (kPipe)
+        fetch(y0, p.plane, kScalarBase ? lane_bytes(x, sizeof(S)) : 0u,
+              kScalarBase ? lane_bytes(x, sizeof(Rec16)) : 0u, hA);
+    if constexpr (kPipe) {
+        // two rows per iteration: the register sets swap roles without copies
+        static_assert(R % 2 == 0, "rows per thread come in pairs");
+#pragma unroll 1
+        for (int r = 0; r < R; r += 2) {
+            if (!row(r, hA, hB)) break;
+            if (!row(r + 1, hB, hA)) break;
+        }
+    } else {
+        // one copy of the (large) row body: D2Q37 FP16-NEQ 8192 x 2048 ran
+        // 1.14 ms with two copies against 0.91 ms with one
+#pragma unroll 1
+        for (int r = 0; r < R; ++r)
+            if (!row(r, hA, hB)) break;
     }
 }
 

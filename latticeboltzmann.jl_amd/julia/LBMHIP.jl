@@ -292,6 +292,51 @@ function initialize_on_device(strategy::LatticeBoltzmann.IterativeInitialization
     fs
 end
 
+# initialize(AnalyticalEquilibrium() / AnalyticalEquilibriumAndOffEquilibrium(),
+# q, problem) (initial_conditions.jl:7-22) with the per-node arithmetic on the
+# device: the problem's own field functions are evaluated here, node by node,
+# as initial_condition evaluates them (problems.jl:98-106), and
+# lbm_initialize_analytical forms the Hermite-sum equilibrium and the
+# off-equilibrium part (analytical_equilibrium.jl:8-17,
+# analytical_offequilibrium.jl:10-87).
+const INIT_EQUILIBRIUM, INIT_OFFEQ_TGV, INIT_OFFEQ = 0, 1, 2
+
+function initialize_on_device(strategy::Union{LatticeBoltzmann.AnalyticalEquilibrium,
+                                              LatticeBoltzmann.AnalyticalEquilibriumAndOffEquilibrium},
+                              q::Quadrature, problem; storage::Symbol = :f64)
+    nx, ny = problem.NX, problem.NY
+    x_range, y_range = LatticeBoltzmann.range(problem)
+    ρ, ux, uy, T = (Array{Float64}(undef, nx, ny) for _ in 1:4)
+    offeq = strategy isa LatticeBoltzmann.AnalyticalEquilibriumAndOffEquilibrium
+    G = offeq ? Array{Float64}(undef, nx, ny, 4) : nothing
+    tgv = problem isa LatticeBoltzmann.TGV
+    for x_idx in 1:nx, y_idx in 1:ny
+        x, y = x_range[x_idx], y_range[y_idx]
+        ρ[x_idx, y_idx] = LatticeBoltzmann.lattice_density(q, problem, x, y)
+        u = LatticeBoltzmann.lattice_velocity(q, problem, x, y)
+        ux[x_idx, y_idx], uy[x_idx, y_idx] = u[1], u[2]
+        T[x_idx, y_idx] = LatticeBoltzmann.lattice_temperature(q, problem, x, y)
+        if offeq
+            ∇u = tgv ? LatticeBoltzmann.velocity_gradient(problem, x, y, 0.0) :
+                 problem.u_max^2 * LatticeBoltzmann.velocity_gradient(problem, x, y, 0.0)
+            for k in 1:4
+                G[x_idx, y_idx, k] = ∇u[k]
+            end
+        end
+    end
+    mode = !offeq ? INIT_EQUILIBRIUM : (tgv ? INIT_OFFEQ_TGV : INIT_OFFEQ)
+    τ = q.speed_of_sound_squared * LatticeBoltzmann.lattice_viscosity(problem)
+    factor = problem.domain_size[1] * problem.domain_size[2]
+    f = HipLattice(q, nx, ny; storage = storage)
+    Gp = offeq ? G : C_NULL
+    GC.@preserve ρ ux uy T G check(ccall((:lbm_initialize_analytical, libpath), Cint,
+                                         (Ptr{Cvoid}, Cint, Ptr{Cdouble}, Ptr{Cdouble}, Ptr{Cdouble},
+                                          Ptr{Cdouble}, Ptr{Cdouble}, Cdouble, Cdouble),
+                                         f.handle, mode, ρ, ux, uy, T, Gp, τ, factor),
+                                   "lbm_initialize_analytical")
+    f
+end
+
 # ---- on-device processing ---------------------------------------------------
 # next!(::TrackHydrodynamicErrors) (track_hydrodynamic_errors.jl:52-222) and
 # should_stop! (stopping_criteria.jl:17-115) on a HipLattice: one kernel pass

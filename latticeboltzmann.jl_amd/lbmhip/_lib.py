@@ -23,17 +23,19 @@ NORTH, EAST, SOUTH, WEST = 0, 1, 2, 3
 # every entry point declared in include/lbm_hip.h
 EXPORTS = [
     "lbm_last_error", "lbm_version", "lbm_device_count", "lbm_synchronize",
-    "lbm_quadrature_info",
+    "lbm_clock_sample", "lbm_quadrature_info",
     "lbm_field_create", "lbm_field_destroy", "lbm_field_info", "lbm_field_upload",
-    "lbm_field_download", "lbm_field_copy",
+    "lbm_field_download", "lbm_field_copy", "lbm_initialize_analytical",
     "lbm_model_create", "lbm_model_destroy", "lbm_model_set_srt", "lbm_model_set_trt",
     "lbm_model_set_mrt", "lbm_model_set_force", "lbm_model_set_bcs", "lbm_model_set_launch",
     "lbm_autotune",
     "lbm_collide", "lbm_stream", "lbm_apply_bcs", "lbm_stream_bc", "lbm_simulate",
-    "lbm_simulate_timed", "lbm_moments", "lbm_reduce",
+    "lbm_simulate_timed", "lbm_simulate_launch_times", "lbm_moments", "lbm_reduce",
     "lbm_batch_create", "lbm_batch_destroy", "lbm_batch_info", "lbm_batch_upload",
     "lbm_batch_download", "lbm_batch_set_launch", "lbm_batch_set_nodes_per_thread",
     "lbm_batch_get_launch", "lbm_batch_autotune", "lbm_batch_simulate",
+    "lbm_snapshots_create", "lbm_snapshots_destroy", "lbm_snapshot_take", "lbm_snapshots_wait",
+    "lbm_snapshot_get",
     "lbm_adjoint_create", "lbm_adjoint_destroy", "lbm_adjoint_info", "lbm_adjoint_loss",
     "lbm_adjoint_gradient",
     "lbm_expected_create", "lbm_expected_destroy", "lbm_expected_upload", "lbm_track_errors",
@@ -92,6 +94,7 @@ def lib() -> ctypes.CDLL:
     L.lbm_version.restype = i
     L.lbm_device_count.restype = i
     L.lbm_synchronize.restype = i
+    L.lbm_clock_sample.argtypes = [ctypes.POINTER(ctypes.c_uint64), dp]
     L.lbm_quadrature_info.argtypes = [i, ip, dp, dp, dp, dp, ip, ip]
     L.lbm_field_create.argtypes = [ctypes.POINTER(vp), i, i, i, i, d]
     L.lbm_field_destroy.argtypes = [vp]
@@ -99,6 +102,7 @@ def lib() -> ctypes.CDLL:
     L.lbm_field_upload.argtypes = [vp, dp]
     L.lbm_field_download.argtypes = [vp, dp]
     L.lbm_field_copy.argtypes = [vp, vp]
+    L.lbm_initialize_analytical.argtypes = [vp, i, dp, dp, dp, dp, dp, d, d]
     L.lbm_model_create.argtypes = [ctypes.POINTER(vp), i]
     L.lbm_model_destroy.argtypes = [vp]
     L.lbm_model_set_srt.argtypes = [vp, d]
@@ -112,8 +116,14 @@ def lib() -> ctypes.CDLL:
         getattr(L, name).argtypes = [vp, vp, vp]
     L.lbm_simulate.argtypes = [vp, vp, vp, i64]
     L.lbm_simulate_timed.argtypes = [vp, vp, vp, i64, dp, ctypes.POINTER(i64)]
+    L.lbm_simulate_launch_times.argtypes = [vp, vp, vp, i64, dp, ctypes.POINTER(i64)]
     L.lbm_moments.argtypes = [vp, d, dp, dp, dp, dp, dp, dp]
     L.lbm_reduce.argtypes = [vp, dp]
+    L.lbm_snapshots_create.argtypes = [ctypes.POINTER(vp), i, i, i, i, ctypes.c_char_p]
+    L.lbm_snapshots_destroy.argtypes = [vp]
+    L.lbm_snapshot_take.argtypes = [vp, vp, i64]
+    L.lbm_snapshots_wait.argtypes = [vp, ip]
+    L.lbm_snapshot_get.argtypes = [vp, i, ctypes.POINTER(i64), dp]
     L.lbm_batch_create.argtypes = [ctypes.POINTER(vp), i, i, i, ip, ip, dp, dp, d]
     L.lbm_batch_destroy.argtypes = [vp]
     L.lbm_batch_info.argtypes = [vp, ip, ctypes.POINTER(i64), ctypes.POINTER(i64)]

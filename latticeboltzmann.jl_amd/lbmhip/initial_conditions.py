@@ -1,11 +1,13 @@
-"""Host-side initial populations (src/initial_conditions.jl:7-22, 182-191).
+"""Initial populations (src/initial_conditions.jl:7-22, initial_conditions/).
 
-``equilibrium_populations(q, ρ, ux, uy)`` fills f[x, y, :] with the truncated
-Hermite equilibrium of each node, the AnalyticalEquilibrium strategy
-(initial_conditions/analytical_equilibrium.jl).  The reference evaluates it
-with the generic Hermite sum (velocity_distribution_function/hermite.jl:10-33);
-the closed form used here is the same polynomial and agrees to rounding.
-Initialization is one-shot host work, outside the timed step.
+``initialize(strategy, q, problem)`` runs the reference's strategies on the
+device: AnalyticalEquilibrium and AnalyticalEquilibriumAndOffEquilibrium per
+node from the problem's lattice fields (lbm_initialize_analytical, the
+Hermite sum of velocity_distribution_function/hermite.jl:10-33 in the
+reference's order), and the iterative initialization of Mei et al.
+``equilibrium_populations(q, ρ, ux, uy)`` is a host convenience for tests and
+benchmarks: the closed-form equilibrium polynomial, equal to the Hermite sum
+to rounding.
 """
 from __future__ import annotations
 
@@ -82,20 +84,70 @@ def initialize_mei(strategy: IterativeInitializationMeiEtAl, q: Quadrature, prob
 
 class AnalyticalEquilibrium:
     """AnalyticalEquilibrium() (initial_conditions/analytical_equilibrium.jl):
-    f0 = feq(rho(x), u(x)) of the problem's analytic fields."""
+    f0 = feq(rho(x), u(x), T(x)) of the problem's analytic fields, by the
+    Hermite sum of hermite_based_equilibrium! (hermite.jl:10-33)."""
+
+
+class AnalyticalEquilibriumAndOffEquilibrium:
+    """AnalyticalEquilibriumAndOffEquilibrium()
+    (initial_conditions/analytical_offequilibrium.jl:9): the analytic
+    equilibrium plus the first-order non-equilibrium part built from the
+    problem's velocity gradient."""
+
+
+# lbm_initialize_analytical strategies (include/lbm_hip.h)
+INIT_EQUILIBRIUM, INIT_OFFEQ_TGV, INIT_OFFEQ = 0, 1, 2
+
+
+def initialize_analytical(q: Quadrature, rho, ux, uy, T=None, grad=None,
+                          strategy: int = INIT_EQUILIBRIUM, tau: float = 0.0,
+                          factor: float = 1.0, storage: str = "f64"):
+    """The analytic initial conditions computed on the device from (nx, ny)
+    planes of lattice density, velocity and (optionally) temperature, and for
+    the off-equilibrium strategies the (nx, ny, 2, 2) velocity gradient
+    G[..., a, b] as the reference passes it; tau = s nu.  Returns a
+    HipLattice in ``storage``."""
+    from . import _lib
+    from .lattice import HipLattice
+
+    rho = np.asfortranarray(rho, dtype=np.float64)
+    nx, ny = rho.shape
+    ux = np.asfortranarray(np.broadcast_to(ux, (nx, ny)), dtype=np.float64)
+    uy = np.asfortranarray(np.broadcast_to(uy, (nx, ny)), dtype=np.float64)
+    Tp = None if T is None else np.asfortranarray(np.broadcast_to(T, (nx, ny)), dtype=np.float64)
+    Gp = None
+    if grad is not None:
+        G = np.asarray(grad, dtype=np.float64)
+        if G.shape != (nx, ny, 2, 2):
+            raise ValueError(f"velocity gradient shape {G.shape} is not ({nx}, {ny}, 2, 2)")
+        Gp = np.asfortranarray(G.reshape(nx, ny, 4, order="F"))
+    f = HipLattice(q, nx, ny, storage=storage)
+    _lib.check(_lib.lib().lbm_initialize_analytical(
+        f.handle, int(strategy), _lib.dptr(rho), _lib.dptr(ux), _lib.dptr(uy),
+        None if Tp is None else _lib.dptr(Tp), None if Gp is None else _lib.dptr(Gp),
+        float(tau), float(factor)), "lbm_initialize_analytical")
+    return f
 
 
 def initialize(strategy, q: Quadrature, problem, storage: str = "f64"):
     """initialize(strategy, q, problem) (initial_conditions.jl:7-22,
-    mei_et_al.jl:13-41) returning the populations on the device: the analytic
-    equilibrium is formed on the host (``problem.initial_populations(q)``) and
-    uploaded once; the iterative initialization runs on the device."""
-    from .lattice import HipLattice
-
+    mei_et_al.jl:13-41) returning the populations on the device.  The problem
+    supplies its analytic lattice fields (``lattice_fields(q)``: density,
+    velocity and temperature planes; ``offequilibrium_inputs()``: the
+    strategy variant, the velocity gradient and the area factor for the
+    off-equilibrium start); the per-node initial conditions are evaluated on
+    the device (lbm_initialize_analytical); the iterative initialization runs
+    on the device."""
     if isinstance(strategy, IterativeInitializationMeiEtAl):
         if storage != "f64":
             raise ValueError("the iterative initialization runs on FP64 storage only")
         return initialize_mei(strategy, q, problem)[0]
     if isinstance(strategy, AnalyticalEquilibrium):
-        return HipLattice.from_numpy(q, problem.initial_populations(q), storage=storage)
+        rho, ux, uy, T = problem.lattice_fields(q)
+        return initialize_analytical(q, rho, ux, uy, T, storage=storage)
+    if isinstance(strategy, AnalyticalEquilibriumAndOffEquilibrium):
+        rho, ux, uy, T = problem.lattice_fields(q)
+        mode, grad, factor = problem.offequilibrium_inputs()
+        tau = q.speed_of_sound_squared * problem.lattice_viscosity()
+        return initialize_analytical(q, rho, ux, uy, T, grad, mode, tau, factor, storage=storage)
     raise TypeError(f"unsupported initialization strategy {strategy!r}")

@@ -178,3 +178,22 @@ def autotune(model: LatticeBoltzmannModel, trial_steps: int = 20) -> dict:
     m._launch_key = model.launch
     return {"block": b.value, "rows_per_block": r.value, "nodes_per_thread": v.value,
             "nontemporal": n.value, "us_per_step": us.value}
+
+
+def simulate_launch_times(model: LatticeBoltzmannModel, nsteps: int):
+    """As ``simulate(model, range(nsteps))`` without processing, with a HIP
+    event before every fused launch: returns the per-launch milliseconds
+    (nsteps - 1 values), for the spread of a kernel's duration over a run."""
+    import ctypes
+
+    import numpy as np
+
+    m = native_model(model.quadrature).configure(model.collision_model, model.f_stream,
+                                                 model.boundary_conditions, 0.0)
+    m.set_launch(model.launch)
+    out = np.zeros(max(int(nsteps) - 1, 1))
+    nl = ctypes.c_int64()
+    _lib.check(_lib.lib().lbm_simulate_launch_times(m.handle, model.f_stream.handle,
+                                                    model.f_collision.handle, int(nsteps),
+                                                    _lib.dptr(out), ctypes.byref(nl)), "simulate")
+    return out[:nl.value]

@@ -209,3 +209,33 @@ def stream_bc_(q: Quadrature, bcs: Sequence[BoundaryCondition], f: HipLattice,
 
 def synchronize() -> None:
     _lib.check(_lib.lib().lbm_synchronize(), "lbm_synchronize")
+
+
+def clock_sample():
+    """Shader-clock and constant-rate counters of every XCD, sampled on the
+    library's stream after the work issued so far: (rows of (XCC id, shader
+    cycles, constant ticks), the constant counter's rate in MHz)."""
+    import ctypes
+
+    import numpy as np
+
+    out = np.zeros(3 * 64, dtype=np.uint64)
+    mhz = ctypes.c_double()
+    _lib.check(_lib.lib().lbm_clock_sample(out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+                                           ctypes.byref(mhz)), "lbm_clock_sample")
+    return out.reshape(64, 3), mhz.value
+
+
+def average_clock_mhz(before, after):
+    """Average shader clock (MHz) over the device work between two
+    ``clock_sample`` results: per XCD, shader cycles over elapsed constant
+    ticks, averaged over the XCDs seen in both samples."""
+    (a, tick), (b, _) = before, after
+    rates = []
+    for xcc in sorted(set(int(v) for v in a[:, 0]) & set(int(v) for v in b[:, 0])):
+        ra = a[a[:, 0] == xcc][0]
+        rb = b[b[:, 0] == xcc][0]
+        dt = float(rb[2]) - float(ra[2])
+        if dt > 0:
+            rates.append((float(rb[1]) - float(ra[1])) / dt * tick)
+    return float(sum(rates) / len(rates)) if rates else None

@@ -136,6 +136,30 @@ class TaylorGreenVortex:
         pf = q.speed_of_sound_squared * self.u_max ** 2 * (-(1 / 4) * self.rho_0 * d ** 2)
         return (1.0, pf, 0.0, d, 0.0, d, 1.0, pf, 0.0, d, 0.0, d, 0.0, d, 0.0, d)
 
+    # -- what the initial conditions read (problems.jl:98-106) -----------------
+    def lattice_fields(self, q: Quadrature):
+        """lattice_density, lattice_velocity = u_max velocity and
+        lattice_temperature = pressure / density, as (NX, NY) planes."""
+        p = self.pressure(q, 0.0)
+        rho = self.density(q, 0.0)
+        ux, uy = self.velocity(0.0)
+        return rho, self.u_max * ux, self.u_max * uy, p / rho
+
+    def velocity_gradient(self, time: float = 0.0) -> np.ndarray:
+        """decay [u_x v_x; u_y v_y] (taylor_green_vortex.jl:87-104) as
+        (NX, NY, 2, 2), G[..., a, b]."""
+        X, Y = self._grid()
+        d = self.decay(time)
+        u_x, v_x, u_y, v_y = (d * g for g in self._gradient_shape(X, Y))
+        return np.stack([np.stack([u_x, v_x], axis=-1), np.stack([u_y, v_y], axis=-1)], axis=-2)
+
+    def offequilibrium_inputs(self):
+        """The generic off-equilibrium start (analytical_offequilibrium.jl:10-48):
+        gradient u_max^2 velocity_gradient and factor = the domain area."""
+        from .initial_conditions import INIT_OFFEQ
+        return (INIT_OFFEQ, self.u_max ** 2 * self.velocity_gradient(0.0),
+                self.domain_size[0] * self.domain_size[1])
+
     # -- initial populations (AnalyticalEquilibrium, problems.jl:136-143) -------
     def initial_populations(self, q: Quadrature) -> np.ndarray:
         rho = self.density(q, 0.0)
@@ -156,3 +180,86 @@ class TaylorGreenVortex:
         ux, uy = self.velocity(0.0)
         k = self.u_max * self.delta_t() * 2 * self.viscosity()
         return np.stack([k * ux, k * uy], axis=-1)
+
+
+class TGV:
+    """TGV(q, τ, scale = 2, NX = 16 scale, NY = NX, u_max = 0.02 / scale)
+    (problems/second_order_convergence.jl:1-131): the Taylor-Green vortex of
+    the convergence study, in lattice units (u_max = 1, domain (1, 1), node
+    coordinates x NX), with its own equilibrium + off-equilibrium start
+    (analytical_offequilibrium.jl:50-87)."""
+
+    kind = "tgv"
+
+    def __init__(self, q: Quadrature, τ: float, scale: int = 2, NX: int = None, NY: int = None,
+                 u_max: float = None):
+        self.q = q
+        self.NX = 16 * scale if NX is None else int(NX)
+        self.NY = self.NX if NY is None else int(NY)
+        self.u_0 = 0.02 / scale if u_max is None else float(u_max)
+        self.u_max = 1.0
+        self.rho_0 = 1.0
+        self.τ = float(τ)
+        self.ν = (self.τ - 0.5) / q.speed_of_sound_squared
+        self.domain_size = (1.0, 1.0)
+
+    def lattice_viscosity(self) -> float:
+        return self.ν
+
+    def _grid(self):
+        dx, dy = 1.0 / self.NX, 1.0 / self.NY
+        x = np.linspace(dx / 2, 1.0 - dx / 2, self.NX) * self.NX
+        y = np.linspace(dy / 2, 1.0 - dy / 2, self.NY) * self.NY
+        return np.meshgrid(x, y, indexing="ij")
+
+    def _k(self):
+        return 2 * math.pi / self.NX, 2 * math.pi / self.NY
+
+    def _td(self):
+        kx, ky = self._k()
+        return 1.0 / (self.ν * (kx * kx + ky * ky))
+
+    def density(self, q: Quadrature, t: float = 0.0) -> np.ndarray:
+        X, Y = self._grid()
+        kx, ky = self._k()
+        shape = (ky / kx) * np.cos(2 * kx * X) + (kx / ky) * np.cos(2 * ky * Y)
+        return self.rho_0 * (1.0 - q.speed_of_sound_squared * (self.u_0 ** 2 / 4) * shape
+                             * np.exp(-2 * t / self._td()))
+
+    def pressure(self, q: Quadrature, t: float = 0.0) -> np.ndarray:
+        X, Y = self._grid()
+        kx, ky = self._k()
+        shape = (ky / kx) * np.cos(2 * kx * X) + (kx / ky) * np.cos(2 * ky * Y)
+        return self.rho_0 - (q.speed_of_sound_squared * 1.0 * (self.u_0 ** 2 / 4) * shape
+                             * np.exp(-2 * t / self._td()))
+
+    def velocity(self, t: float = 0.0):
+        X, Y = self._grid()
+        kx, ky = self._k()
+        d = self.u_0 * np.exp(-t / self._td())
+        return (d * (-math.sqrt(ky / kx) * np.cos(kx * X) * np.sin(ky * Y)),
+                d * (math.sqrt(kx / ky) * np.sin(kx * X) * np.cos(ky * Y)))
+
+    def velocity_gradient(self, t: float = 0.0) -> np.ndarray:
+        X, Y = self._grid()
+        kx, ky = self._k()
+        u_x = math.sqrt(ky * kx) * np.sin(kx * X) * np.sin(ky * Y)
+        v_y = -math.sqrt(ky * kx) * np.sin(kx * X) * np.sin(ky * Y)
+        u_y = -math.sqrt(ky ** 3 / kx) * np.cos(kx * X) * np.cos(ky * Y)
+        v_x = math.sqrt(kx ** 3 / ky) * np.cos(kx * X) * np.cos(ky * Y)
+        d = np.exp(-t / self._td()) * self.u_0
+        return d * np.stack([np.stack([u_x, v_x], axis=-1), np.stack([u_y, v_y], axis=-1)],
+                            axis=-2)
+
+    def lattice_fields(self, q: Quadrature):
+        rho = self.density(q)
+        ux, uy = self.velocity()
+        return rho, ux, uy, self.pressure(q) / rho
+
+    def offequilibrium_inputs(self):
+        from .initial_conditions import INIT_OFFEQ_TGV
+        return INIT_OFFEQ_TGV, self.velocity_gradient(0.0), 1.0
+
+    def lattice_velocity_field(self) -> np.ndarray:
+        ux, uy = self.velocity()
+        return np.stack([ux, uy], axis=-1)

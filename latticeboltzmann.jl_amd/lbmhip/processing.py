@@ -17,6 +17,7 @@ from __future__ import annotations
 
 import ctypes
 import math
+import os
 from typing import List, NamedTuple, Optional, Sequence
 
 import numpy as np
@@ -306,28 +307,71 @@ class TrackHydrodynamicErrors(ProcessingMethod):
 class TakeSnapshots(ProcessingMethod):
     """TakeSnapshots(problem, every_t) (take_snapshots.jl:3-29): at every
     ``every_t``-th step (an int) or at the listed steps (a list) the
-    populations are copied.  The copies stay on the device (a device-to-device
-    copy kernel into a new HipLattice; HBM holds many of them) and are
-    downloaded only when read (:meth:`arrays`).  Plotting is not done."""
+    populations are copied (``snapshot = copy(f_in)``).  The copy goes through
+    one of ``device_slots`` staging buffers in HBM to pinned host memory on a
+    second stream (lbm_snapshot_take), so the step does not wait for the
+    host; with ``directory`` each snapshot is also written there as
+    snapshot_<t>.npy.  :meth:`arrays` gives the (nx, ny, Q) Float64 arrays.
+    Plotting is not done."""
 
-    def __init__(self, problem, every_t):
+    def __init__(self, problem, every_t, device_slots: int = 2, directory: str = None):
         self.problem = problem
         self.every_t = every_t
-        self.snapshots: List[HipLattice] = []
+        self.device_slots = int(device_slots)
+        self.directory = directory
         self.timesteps: List[int] = []
+        self._h = None
+        self._shape = None
 
     def needs_step(self, t: int) -> bool:
         if isinstance(self.every_t, int):
             return t % self.every_t == 0
         return t in self.every_t
 
+    def _handle(self, q: Quadrature, f: HipLattice):
+        import ctypes
+        if self._h is None:
+            h = ctypes.c_void_p()
+            d = None if self.directory is None else os.fsencode(self.directory)
+            _lib.check(_lib.lib().lbm_snapshots_create(ctypes.byref(h), q.id, f.nx, f.ny,
+                                                       self.device_slots, d),
+                       "lbm_snapshots_create")
+            self._h = h
+            self._shape = (f.nx, f.ny, q.Q)
+        return self._h
+
     def next_(self, q: Quadrature, f: HipLattice, t: int) -> bool:
         if not self.needs_step(t):
             return False
-        self.snapshots.append(f.copy())
+        _lib.check(_lib.lib().lbm_snapshot_take(self._handle(q, f), f.handle, int(t)),
+                   "lbm_snapshot_take")
         self.timesteps.append(int(t))
         return False
 
+    def wait(self) -> int:
+        import ctypes
+        if self._h is None:
+            return 0
+        n = ctypes.c_int()
+        _lib.check(_lib.lib().lbm_snapshots_wait(self._h, ctypes.byref(n)), "lbm_snapshots_wait")
+        return n.value
+
     def arrays(self) -> List[np.ndarray]:
-        """the snapshots as (nx, ny, Q) Float64 host arrays"""
-        return [s.to_numpy() for s in self.snapshots]
+        """the snapshots as (nx, ny, Q) Float64 host arrays, in the order taken"""
+        import ctypes
+        out = []
+        for k in range(self.wait()):
+            a = np.empty(self._shape, order="F")
+            t = ctypes.c_int64()
+            _lib.check(_lib.lib().lbm_snapshot_get(self._h, k, ctypes.byref(t), _lib.dptr(a)),
+                       "lbm_snapshot_get")
+            out.append(a)
+        return out
+
+    def __del__(self):
+        if getattr(self, "_h", None) is not None:
+            try:
+                _lib.lib().lbm_snapshots_destroy(self._h)
+            except Exception:
+                pass
+            self._h = None

@@ -112,3 +112,21 @@ def test_batch_rejects_other_quadratures():
     import lbmhip as L
     with pytest.raises(L.LbmError, match="D2Q9"):
         L.LatticeBatch(L.D2Q17(), [(4, 4)], [0.8])
+
+
+def test_batch_graph_after_tile_table_rebuilt(oracle_lib):
+    """A captured step graph holds the tile table's address: changing the
+    nodes per thread rebuilds the table, which must drop the graph, including
+    a later switch back to the shape the graph was captured with."""
+    b, f0, ts, ta = _batch("f64")
+    b.set_launch(128, 0, nodes_per_thread=1)
+    b.simulate(6, graph_chunk=6)
+    b.set_launch(128, 0, nodes_per_thread=2)
+    b.simulate(6, graph_chunk=6)
+    b.set_launch(128, 0, nodes_per_thread=1)
+    b.simulate(6, graph_chunk=6)
+    for k, f in enumerate(f0):
+        S, C = f.copy(order="F"), f.copy(order="F")
+        oracle_lib.simulate(9, S, C, 1, [ts[k], ta[k]], 18)
+        assert np.array_equal(b.download(k, "f_stream"), S), k
+        assert np.array_equal(b.download(k, "f_collision"), C), k

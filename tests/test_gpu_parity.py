@@ -363,41 +363,47 @@ def test_vec_requires_divisible_nx():
 _LP_TOL = {"f32": 2e-5, "f16dev": 5e-3, "bf16dev": 5e-2, "f16neq": 1e-4, "bf16neq": 5e-4}
 
 
-@pytest.mark.parametrize("qid", [9, 17, 37])
-@pytest.mark.parametrize("storage", ["f32", "f16dev", "bf16dev", "f16neq", "bf16neq"])
-@pytest.mark.parametrize("vec", [1, 2, 4, 8, 16])
+def _max_vec(Q, storage):
+    """Largest nodes per thread the library instantiates (max_vec of
+    lbm_kernels.h): rows for the non-equilibrium storages, x-runs otherwise."""
+    if storage.endswith("neq"):
+        return 16
+    cap = 4 if storage == "f32" else 8
+    return cap if Q <= 9 else (min(cap, 4) if Q <= 17 else 2)
+
+
+_LP_CASES = [(qid, st, vec) for qid in (9, 17, 37)
+             for st in ("f32", "f16dev", "bf16dev", "f16neq", "bf16neq")
+             for vec in (1, 2, 4, 8, 16)
+             if vec <= _max_vec({9: 9, 17: 17, 37: 37}[qid], st)]
+
+
+@pytest.mark.parametrize("qid,storage,vec", _LP_CASES)
 @pytest.mark.parametrize("coll", [0, 1])
-def test_reduced_storage_tracks_fp64(qid, storage, vec, coll):
-    """FP32 arithmetic on FP32 / FP16-deviation / BF16-deviation storage stays
-    within the storage's precision of the FP64 run (relative L2 velocity
-    error after 20 steps of a Couette + force case)."""
+def test_reduced_storage_tracks_oracle(oracle_lib, qid, storage, vec, coll):
+    """FP32 arithmetic on FP32 / FP16-deviation / BF16-deviation /
+    non-equilibrium storage stays within the storage's precision of the
+    oracle's FP64 run (relative L2 velocity error after 20 steps of a Couette
+    + force case), for every launch shape the library instantiates."""
+    from oracle import lbm_numpy as ln
     L = _L()
     q = _q(qid)
     nx, ny = 32, 12
-    if vec > 1 and vec * q.Q > 80 and not (q.Q <= 9) and vec > 2:
-        pytest.skip("not instantiated (register budget)")
-    if q.Q > 17 and vec > 2:
-        pytest.skip("not instantiated (register budget)")
-    if q.Q > 9 and vec > 4:
-        pytest.skip("not instantiated (register budget)")
-    if storage == "f32" and vec > 4:
-        pytest.skip("not instantiated for 4-byte storage")
-    if vec == 16 and not (storage.endswith("neq") and q.Q <= 9):
-        pytest.skip("16 nodes per thread: rows of the non-equilibrium storages only")
     f0 = random_populations(qid, nx, ny, seed=17, noise=1e-4)
-    cm, _ = _cm(coll, qid, force_for("const", nx, ny))
-    bcs = to_lbmhip(bc_dicts("couette", nx, ny))
-    ref = L.LatticeBoltzmannModel.from_populations(q, f0, cm, bcs)
-    L.simulate(ref, range(20))
-    lp = L.LatticeBoltzmannModel.from_populations(q, f0, cm, bcs, storage=storage)
+    F = force_for("const", nx, ny)
+    cm, taus = _cm(coll, qid, F)
+    bcd = bc_dicts("couette", nx, ny)
+    S, C = f0.copy(order="F"), f0.copy(order="F")
+    oracle_lib.simulate(qid, S, C, coll, taus, 20, force=F, bcs=bcd)
+    rho, ux, uy = ln.moments_field(ln.by_id(qid), S)
+    lp = L.LatticeBoltzmannModel.from_populations(q, f0, cm, to_lbmhip(bcd), storage=storage)
     lp.launch = (64, vec, 0)
     L.simulate(lp, range(20))
-    a = L.moments(ref.f_stream, stress=False)
     b = L.moments(lp.f_stream, stress=False)
-    num = np.sqrt(np.sum((a.ux - b.ux) ** 2 + (a.uy - b.uy) ** 2))
-    den = np.sqrt(np.sum(a.ux ** 2 + a.uy ** 2))
+    num = np.sqrt(np.sum((ux - b.ux) ** 2 + (uy - b.uy) ** 2))
+    den = np.sqrt(np.sum(ux ** 2 + uy ** 2))
     assert num / den < _LP_TOL[storage], (storage, num / den)
-    np.testing.assert_allclose(b.rho, a.rho, rtol=_LP_TOL[storage] * 0.1)
+    np.testing.assert_allclose(b.rho, rho, rtol=_LP_TOL[storage] * 0.1)
 
 
 @pytest.mark.parametrize("storage", ["f32", "f16dev", "bf16dev", "f16neq", "bf16neq"])
@@ -434,6 +440,43 @@ def test_autotune_keeps_state(oracle_lib):
     oracle_lib.simulate(9, S, C, 1, [model.collision_model.τ_symmetric,
                                       model.collision_model.τ_asymmetric], 5)
     _assert_same(model.f_stream.to_numpy(), S, "tuned launch shape")
+
+
+def test_autotune_validates_model():
+    """The autotuner runs the fused kernel on the model's force field and BC
+    list: it applies the same checks as simulate before any launch."""
+    import ctypes
+
+    import lbmhip._lib as lib
+
+    L = _L()
+    q = L.D2Q9()
+    f0 = taylor_green_vortex_pops(q, 32)
+    # through the C ABI: a model whose force field belongs to a 16 x 32 lattice
+    C = lib.lib()
+    m = ctypes.c_void_p()
+    assert C.lbm_model_create(ctypes.byref(m), 9) == 0
+    try:
+        assert C.lbm_model_set_srt(m, 0.8) == 0
+        wrong = np.zeros((16, 32, 2), order="F")
+        assert C.lbm_model_set_force(m, lib.FORCE_FIELD, lib.dptr(wrong), 16, 32) == 0
+        f = L.HipLattice.from_numpy(q, f0)
+        b, r, v, n = (ctypes.c_int() for _ in range(4))
+        us = ctypes.c_double()
+        rc = C.lbm_autotune(m, f.handle, 1, ctypes.byref(b), ctypes.byref(r), ctypes.byref(v),
+                            ctypes.byref(n), ctypes.byref(us))
+        assert rc != 0 and b"force field" in C.lbm_last_error()
+    finally:
+        C.lbm_model_destroy(m)
+    bcs = [L.BounceBack(L.North(), (1, 33), (1, 32))]
+    model = L.LatticeBoltzmannModel.from_populations(q, f0, L.SRT(0.8), bcs)
+    with pytest.raises(L.LbmError, match="outside"):
+        L.autotune(model, trial_steps=1)
+    u = np.zeros((32, 32, 2), order="F")
+    model = L.LatticeBoltzmannModel.from_populations(
+        q, f0, L.IterativeInitializationCollisionModel(1.0, u))
+    with pytest.raises(L.LbmError, match="fused step"):
+        L.autotune(model, trial_steps=1)
 
 
 @pytest.mark.parametrize("storage", ["f16neq", "bf16neq"])
@@ -521,7 +564,7 @@ def test_neq_row_marching_matches_single_node(storage, rows, shape, bcname):
 
 @pytest.mark.parametrize("qid", [13, 17, 21, 37])
 @pytest.mark.parametrize("storage", ["f16neq", "bf16neq"])
-@pytest.mark.parametrize("rows", [2, 8])
+@pytest.mark.parametrize("rows", [2, 4, 8, 16])
 @pytest.mark.parametrize("shape", [(5, 5), (7, 9), (9, 7), (40, 23), (70, 31)])
 @pytest.mark.parametrize("bcname", ["periodic", "couette"])
 def test_neq_row_marching_wide_stencils(qid, storage, rows, shape, bcname):

@@ -236,3 +236,53 @@ def test_take_snapshots(oracle_lib, every):
         oracle_lib.simulate(9, S, C, 1, [0.7, 1.2], t - done)
         done = t
         assert np.array_equal(snap, S), t
+
+
+@pytest.mark.parametrize("slots", [1, 3])
+def test_snapshots_async_slots_and_files(oracle_lib, tmp_path, slots):
+    """More snapshots than staging slots (each slot reused after its host
+    copy), written as .npy files: every array equals the oracle's state at
+    its step, in memory and on disk."""
+    L = _L()
+    q = L.D2Q9()
+    nx, ny = 64, 48
+    f0 = random_populations(9, nx, ny, seed=10)
+    pm = L.TakeSnapshots(None, 2, device_slots=slots, directory=str(tmp_path))
+    m = L.LatticeBoltzmannModel.from_populations(q, f0, L.TRT.fields(0.7, 1.2),
+                                                 processing_method=pm)
+    L.simulate(m, range(0, 12))
+    assert pm.timesteps == [2, 4, 6, 8, 10, 12, 12]
+    snaps = pm.arrays()
+    S = np.asfortranarray(f0.copy())
+    C = np.asfortranarray(f0.copy())
+    done = 0
+    for t, snap in zip(pm.timesteps, snaps):
+        oracle_lib.simulate(9, S, C, 1, [0.7, 1.2], t - done)
+        done = t
+        assert np.array_equal(snap, S), t
+        disk = np.load(tmp_path / f"snapshot_{t:010d}.npy")
+        assert disk.shape == (nx, ny, 9) and disk.flags.f_contiguous
+        assert np.array_equal(disk, S), t
+
+
+@pytest.mark.parametrize("storage", ["f32", "f16neq", "bf16dev"])
+def test_snapshots_reduced_storage(storage):
+    """Snapshots of a reduced-storage field are its decoded (nx, ny, Q) Float64
+    populations, as a download gives them."""
+    L = _L()
+    q = L.D2Q17()
+    nx, ny = 40, 24
+    f0 = random_populations(17, nx, ny, seed=11, noise=1e-4)
+    pm = L.TakeSnapshots(None, [3, 7])
+    m = L.LatticeBoltzmannModel.from_populations(q, f0, L.SRT(0.8), processing_method=pm,
+                                                 storage=storage)
+    ref = L.LatticeBoltzmannModel.from_populations(q, f0, L.SRT(0.8), storage=storage)
+    L.simulate(m, range(0, 7))
+    L.simulate(ref, range(0, 3))
+    a3 = ref.f_stream.to_numpy()
+    L.simulate(ref, range(3, 7))
+    a7 = ref.f_stream.to_numpy()
+    snaps = pm.arrays()
+    assert pm.timesteps == [3, 7, 7]
+    assert np.array_equal(snaps[0], a3)
+    assert np.array_equal(snaps[1], a7) and np.array_equal(snaps[2], a7)
