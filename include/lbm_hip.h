@@ -310,6 +310,16 @@ int lbm_moments(lbm_field f, double tau, double *rho, double *ux, double *uy,
  * out[0] = sum rho, out[1] = sum rho ux, out[2] = sum rho uy,
  * out[3] = sum rho |u|^2, out[4] = number of nodes with a non-finite population */
 int lbm_reduce(lbm_field f, double *out5);
+/* As lbm_reduce, allocating nothing (no hipMalloc/hipFree): no counterpart in
+ * the reference; for callers that cannot allocate device memory (code under
+ * stream capture, tight serving loops, harnesses that forbid allocation).
+ * Its partial sums live in a library-owned static device array of 2048 x 5 +
+ * 8 doubles per GPU (80 KiB, held until the process ends), which covers any
+ * lattice size (the grid is capped at 2048 blocks).  Storages F64, F32,
+ * F16DEV, BF16DEV; the non-equilibrium storages return an error (they need a
+ * decoded copy; use lbm_reduce).  Calls that overlap on one GPU share the
+ * array and race.  Synchronous. */
+int lbm_reduce_noalloc(lbm_field f, double *out5);
 
 /* ---- on-device processing ----------------------------------------------- */
 /* Analytic reference fields for the error norms of TrackHydrodynamicErrors:

@@ -379,6 +379,37 @@ void free_dev(void *&p) {
 
 }  // namespace
 
+// Scratch of lbm_reduce_noalloc: a file-scope device array (each GPU holds
+// its own copy for the life of the process), sized for lbm_reduce's largest
+// grid -- kReduceRows partial rows of 5 sums, then the 5 results (and 3
+// spare).  Host code must not pass the array's name to a kernel (in host code
+// it is a stand-in, not a device address): the device address comes from
+// hipGetSymbolAddress, once per device with that device current.  Calls that
+// overlap on one GPU share it and race.
+namespace {
+constexpr int kReduceRows = 2048;
+constexpr size_t kScratchDoubles = (size_t)kReduceRows * 5 + 8;
+constexpr int kMaxDevices = 64;
+double *g_scratch[kMaxDevices] = {};
+}  // namespace
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+static int scratch_ptr(double **out) {
+    if (int rc = ensure_stream()) return rc;
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(-3, "device index %d out of range", dev);
+    if (!g_scratch[dev]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return fail(-3, "hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+        g_scratch[dev] = static_cast<double *>(ptr);
+    }
+    *out = g_scratch[dev];
+    return 0;
+}
+
 static int lbm_field_copy_raw(void *dst, const void *src, size_t bytes) {
     if (int rc = lbm::dispatch_copy(src, dst, bytes, g_stream)) return fail(-4, "copy kernel failed (%d)", rc);
     return 0;
@@ -1092,6 +1123,31 @@ int lbm_reduce(lbm_field f, double *out5) {
     free_dev(reinterpret_cast<void *&>(ftmp));
     if (e != hipSuccess) return fail(-3, "lbm_reduce: %s", hipGetErrorString(e));
     if (rc) return fail(-4, "reduction kernel failed (%d)", rc);
+    return 0;
+}
+
+int lbm_reduce_noalloc(lbm_field f, double *out5) {
+    if (int rc = check_field(f, "lbm_reduce_noalloc")) return rc;
+    if (!out5) return fail(-1, "lbm_reduce_noalloc: null output");
+    if (is_neq(f))
+        return fail(-2, "lbm_reduce_noalloc: non-equilibrium storages are decoded into a copy of "
+                        "the field first; use lbm_reduce");
+    const lbm::StepParams<double> p = field_params(f);
+    // lbm_reduce's grid: at most kReduceRows blocks, whatever the lattice size
+    const long long blocks = (p.plane + 255) / 256;
+    const int npart = (int)(blocks < kReduceRows ? blocks : kReduceRows);
+    if (npart < 1 || (size_t)npart * 5 + 8 > kScratchDoubles)
+        return fail(-1, "lbm_reduce_noalloc: %d partial rows exceed the %d-row scratch", npart,
+                    kReduceRows);
+    double *sc = nullptr;
+    if (int rc = scratch_ptr(&sc)) return rc;
+    // every partial row and output entry is written before it is read: the
+    // scratch needs no clearing between calls
+    double *part = sc, *out = sc + (size_t)kReduceRows * 5;
+    if (int rc = lbm::dispatch_reduce(f->qid, f->storage, f->d, p, part, npart, out, g_stream))
+        return fail(-4, "reduction kernel failed (%d)", rc);
+    HIP_TRY(hipMemcpyAsync(out5, out, 5 * sizeof(double), hipMemcpyDeviceToHost, g_stream));
+    HIP_TRY(hipStreamSynchronize(g_stream));
     return 0;
 }
 

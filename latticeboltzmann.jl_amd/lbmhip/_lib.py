@@ -31,6 +31,7 @@ EXPORTS = [
     "lbm_autotune",
     "lbm_collide", "lbm_stream", "lbm_apply_bcs", "lbm_stream_bc", "lbm_simulate",
     "lbm_simulate_timed", "lbm_simulate_launch_times", "lbm_moments", "lbm_reduce",
+    "lbm_reduce_noalloc",
     "lbm_batch_create", "lbm_batch_destroy", "lbm_batch_info", "lbm_batch_upload",
     "lbm_batch_download", "lbm_batch_set_launch", "lbm_batch_set_nodes_per_thread",
     "lbm_batch_get_launch", "lbm_batch_autotune", "lbm_batch_simulate",
@@ -119,6 +120,7 @@ def lib() -> ctypes.CDLL:
     L.lbm_simulate_launch_times.argtypes = [vp, vp, vp, i64, dp, ctypes.POINTER(i64)]
     L.lbm_moments.argtypes = [vp, d, dp, dp, dp, dp, dp, dp]
     L.lbm_reduce.argtypes = [vp, dp]
+    L.lbm_reduce_noalloc.argtypes = [vp, dp]
     L.lbm_snapshots_create.argtypes = [ctypes.POINTER(vp), i, i, i, i, ctypes.c_char_p]
     L.lbm_snapshots_destroy.argtypes = [vp]
     L.lbm_snapshot_take.argtypes = [vp, vp, i64]
