@@ -471,6 +471,9 @@ def main():
     ap.add_argument("--n", type=int, default=4096)
     ap.add_argument("--tau", type=float, default=0.6)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--accuracy-steps", type=int, default=ACCURACY_STEPS,
+                    help="untimed steps of the FP64-vs-storage velocity comparison (the "
+                         "workload's 5000; fewer only for profiler runs)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp16", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
@@ -494,6 +497,8 @@ def main():
         print(f"bench.py: liblbm_hip runs on one GPU; --gpus {a.gpus} is not supported "
               "(sharding the lattice as y-slabs is designed, not implemented)", file=sys.stderr)
         sys.exit(2)
+    if a.accuracy_steps < 1:
+        ap.error("--accuracy-steps must be at least 1")
 
     import lbmhip as L
     if L.device_count() < 1:
@@ -529,9 +534,10 @@ def main():
     kavg64 = kms64 / max(nl64, 1)
     gbs64 = bytes64 / (kavg64 * 1e-3) / 1e9
     spread64 = launch_spread(L, m64)
-    print(f"[bench] FP64 accuracy reference: {ACCURACY_STEPS} untimed steps", file=sys.stderr,
+    acc_steps = a.accuracy_steps
+    print(f"[bench] FP64 accuracy reference: {acc_steps} untimed steps", file=sys.stderr,
           flush=True)
-    u64 = velocity_after(L, q, f0, cm, "f64", ACCURACY_STEPS)
+    u64 = velocity_after(L, q, f0, cm, "f64", acc_steps)
 
     result = {
         "metric": METRIC,
@@ -598,16 +604,16 @@ def main():
             gbs16 = bytes16 / (kavg16 * 1e-3) / 1e9
             spread16 = launch_spread(L, m16)
             del m16
-            print(f"[bench] {storage} accuracy: {ACCURACY_STEPS} untimed steps", file=sys.stderr,
+            print(f"[bench] {storage} accuracy: {acc_steps} untimed steps", file=sys.stderr,
                   flush=True)
-            u16 = velocity_after(L, q, f0, cm, storage, ACCURACY_STEPS)
+            u16 = velocity_after(L, q, f0, cm, storage, acc_steps)
             result[key] = {
                 "value": round(mlups16, 1),
                 "unit": "MLUPS",
                 "storage": storage,
                 "ratio_vs_f64": round(mlups16 / mlups64, 3),
                 "rel_l2_velocity_err_vs_f64": rel_l2(u16, u64),
-                "steps_compared": ACCURACY_STEPS,
+                "steps_compared": acc_steps,
                 "ms_per_step": round(wall16 / K * 1e3, 5),
                 "roofline": {"bound": "hbm", "achieved": round(gbs16, 1), "peak": HBM_PEAK_GBS,
                              "unit": "GB/s", "frac": round(gbs16 / HBM_PEAK_GBS, 4),

@@ -84,12 +84,17 @@ int lbm_device_count(void);
 /* Blocks until all work issued by the library has finished. */
 int lbm_synchronize(void);
 
-/* Measurement support; no counterpart in the reference.  Runs a probe on
- * the library's stream, after the work already issued on it, and returns
- * out[3 * 64]: per probe block (XCC id, shader-clock cycles, constant-rate
- * ticks); *tick_mhz (may be NULL) is the constant counter's rate.  Two
- * samples around a stretch of device work give the average shader clock of
- * each XCD over it.  Synchronous. */
+/* Measurement support; no counterpart in the reference.  Runs a probe of
+ * LBM_CLOCK_PROBES blocks (enough to reach every compute unit) on the
+ * library's stream, after the work already issued on it, and returns
+ * out[3 * LBM_CLOCK_PROBES]: per probe block (location key, shader-clock
+ * cycles, constant-rate ticks), the key being (XCC id << 8) | (SE, SH and CU
+ * ids); *tick_mhz (may be NULL) is the constant counter's rate.  The
+ * shader-clock counter runs per compute unit and its counters are not
+ * aligned with each other, so two samples around a stretch of device work
+ * are compared per location key: cycles over ticks is that compute unit's
+ * average clock over the stretch.  Synchronous. */
+#define LBM_CLOCK_PROBES 1024
 int lbm_clock_sample(uint64_t *out, double *tick_mhz);
 
 /* Quadrature constants as the library holds them (c as doubles, 2*Q).

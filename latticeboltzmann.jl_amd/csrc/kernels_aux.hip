@@ -685,12 +685,17 @@ int dispatch_init_analytical(int qid, double *f, const StepParams<double> &p, co
 // constant-rate counter).  Blocks are dealt round-robin over the XCDs, so
 // 64 blocks sample every XCD; the counters of one XCD are compared between two
 // probes.  s_getreg field: HW_REG_XCC_ID (20), bits [3:0].
+// One row per block: where it ran, the shader-clock counter and the
+// constant-rate counter.  The shader-clock counter is not synchronised
+// between compute units, so the location key (XCC_ID bits 3:0, then HW_ID
+// bits 15:8 = CU, SH and SE ids) lets two samples be paired per compute unit.
 __global__ void __launch_bounds__(64) k_clock_probe(unsigned long long *out) {
     if (threadIdx.x != 0) return;
     const unsigned xcc = __builtin_amdgcn_s_getreg((3 << 11) | 20) & 15u;
+    const unsigned hw = __builtin_amdgcn_s_getreg((31 << 11) | 4);
     const unsigned long long t = __builtin_amdgcn_s_memtime();
     const unsigned long long r = __builtin_amdgcn_s_memrealtime();
-    out[3 * blockIdx.x] = xcc;
+    out[3 * blockIdx.x] = (xcc << 8) | ((hw >> 8) & 0xffu);
     out[3 * blockIdx.x + 1] = t;
     out[3 * blockIdx.x + 2] = r;
 }

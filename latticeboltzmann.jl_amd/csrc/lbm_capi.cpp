@@ -437,10 +437,11 @@ int lbm_clock_sample(uint64_t *out, double *tick_mhz) {
     if (!out) return fail(-1, "lbm_clock_sample: null output");
     if (int rc = ensure_stream()) return rc;
     static unsigned long long *d = nullptr;
-    if (!d) HIP_TRY(hipMalloc(&d, 3 * 64 * sizeof(unsigned long long)));
-    if (lbm::dispatch_clock_probe(d, 64, g_stream)) return fail(-4, "clock probe launch failed");
-    HIP_TRY(hipMemcpyAsync(out, d, 3 * 64 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
-                           g_stream));
+    constexpr size_t bytes = 3 * (size_t)LBM_CLOCK_PROBES * sizeof(unsigned long long);
+    if (!d) HIP_TRY(hipMalloc(&d, bytes));
+    if (lbm::dispatch_clock_probe(d, LBM_CLOCK_PROBES, g_stream))
+        return fail(-4, "clock probe launch failed");
+    HIP_TRY(hipMemcpyAsync(out, d, bytes, hipMemcpyDeviceToHost, g_stream));
     HIP_TRY(hipStreamSynchronize(g_stream));
     if (tick_mhz) {
         int dev = 0, khz = 0;

@@ -211,31 +211,47 @@ def synchronize() -> None:
     _lib.check(_lib.lib().lbm_synchronize(), "lbm_synchronize")
 
 
+CLOCK_PROBES = 1024  # LBM_CLOCK_PROBES (include/lbm_hip.h)
+
+
 def clock_sample():
-    """Shader-clock and constant-rate counters of every XCD, sampled on the
-    library's stream after the work issued so far: (rows of (XCC id, shader
-    cycles, constant ticks), the constant counter's rate in MHz)."""
+    """Shader-clock and constant-rate counters sampled on the library's
+    stream after the work issued so far, one row per probe block: (rows of
+    (location key, shader cycles, constant ticks), the constant counter's
+    rate in MHz).  The key names the compute unit the block ran on."""
     import ctypes
 
     import numpy as np
 
-    out = np.zeros(3 * 64, dtype=np.uint64)
+    out = np.zeros(3 * CLOCK_PROBES, dtype=np.uint64)
     mhz = ctypes.c_double()
     _lib.check(_lib.lib().lbm_clock_sample(out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
                                            ctypes.byref(mhz)), "lbm_clock_sample")
-    return out.reshape(64, 3), mhz.value
+    return out.reshape(CLOCK_PROBES, 3), mhz.value
 
 
 def average_clock_mhz(before, after):
     """Average shader clock (MHz) over the device work between two
-    ``clock_sample`` results: per XCD, shader cycles over elapsed constant
-    ticks, averaged over the XCDs seen in both samples."""
+    ``clock_sample`` results.  The shader-clock counters of different compute
+    units are not aligned, so the two samples are paired per location key:
+    each compute unit seen in both gives cycles / ticks * tick rate, and the
+    result is the median over those units (None if none pairs)."""
+    import numpy as np
+
     (a, tick), (b, _) = before, after
+
+    def first(rows):
+        seen = {}
+        for key, cyc, ticks in rows.tolist():
+            if key not in seen or ticks < seen[key][1]:
+                seen[key] = (cyc, ticks)
+        return seen
+
+    sa, sb = first(a), first(b)
     rates = []
-    for xcc in sorted(set(int(v) for v in a[:, 0]) & set(int(v) for v in b[:, 0])):
-        ra = a[a[:, 0] == xcc][0]
-        rb = b[b[:, 0] == xcc][0]
-        dt = float(rb[2]) - float(ra[2])
-        if dt > 0:
-            rates.append((float(rb[1]) - float(ra[1])) / dt * tick)
-    return float(sum(rates) / len(rates)) if rates else None
+    for key in sa.keys() & sb.keys():
+        dt = sb[key][1] - sa[key][1]
+        dc = sb[key][0] - sa[key][0]
+        if dt > 0 and dc > 0:
+            rates.append(dc / dt * tick)
+    return float(np.median(rates)) if rates else None

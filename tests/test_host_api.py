@@ -94,3 +94,20 @@ def test_bench_refuses_multi_gpu():
     r = subprocess.run([sys.executable, os.path.join(repo, "bench.py"), "--gpus", "2"],
                        capture_output=True, text=True)
     assert r.returncode == 2 and "one GPU" in r.stderr
+
+
+def test_average_clock_pairs_samples_per_compute_unit():
+    # Two compute units whose shader-clock counters start far apart: pairing
+    # by location key gives each unit's own rate; pairing across units would
+    # give nonsense (here a negative rate).
+    import numpy as np
+
+    import lbmhip as L
+
+    tick = 100.0
+    a = np.array([[0x0103, 5_000_000_000, 1000], [0x0204, 10, 1000]], dtype=np.uint64)
+    b = np.array([[0x0204, 10 + 2_400_000, 1100], [0x0103, 5_000_000_000 + 2_000_000, 1100],
+                  [0x0305, 7, 1100]], dtype=np.uint64)
+    mhz = L.average_clock_mhz((a, tick), (b, tick))
+    assert mhz == np.median([2_400_000 / 100 * tick, 2_000_000 / 100 * tick])
+    assert L.average_clock_mhz((a[:1], tick), (b[2:], tick)) is None
