@@ -50,11 +50,21 @@ ACCURACY_STEPS = 5000
 SPREAD_LAUNCHES = 200
 
 
-def run_storage(L, q, f0, cm, storage, steps, warmup):
-    """W untimed warm-up steps, then K timed steps (wall clock around them,
-    HIP events around the fused launches), with the shader clock sampled
-    before and after the timed steps."""
+def run_storage(L, q, f0, cm, storage, steps, warmup, accuracy_steps):
+    """The velocity after an untimed ``accuracy_steps``-step run of this
+    storage, then, on a model uploaded before that run, W untimed warm-up
+    steps and K timed steps (wall clock around them, HIP events around the
+    fused launches), with the shader clock sampled before and after the timed
+    steps.  The accuracy run goes first so that the timed launches start from
+    the GPU's sustained state rather than from the idle stretch of a host
+    upload (clock-sensitive kernels run their first launches after idle up to
+    10% slower)."""
     model = L.LatticeBoltzmannModel.from_populations(q, f0, cm, storage=storage)
+    print(f"[bench] {storage}: accuracy run, {accuracy_steps} untimed steps", file=sys.stderr,
+          flush=True)
+    u = velocity_after(L, q, f0, cm, storage, accuracy_steps)
+    print(f"[bench] {storage}: {warmup} warmup + {steps} timed steps", file=sys.stderr,
+          flush=True)
     if warmup:
         L.simulate(model, range(warmup))
     L.synchronize()
@@ -64,7 +74,7 @@ def run_storage(L, q, f0, cm, storage, steps, warmup):
     L.synchronize()
     wall = time.perf_counter() - t0
     c1 = L.clock_sample()
-    return model, wall, kernel_ms, nlaunch, L.average_clock_mhz(c0, c1)
+    return model, wall, kernel_ms, nlaunch, L.average_clock_mhz(c0, c1), u
 
 
 def launch_spread(L, model, launches=SPREAD_LAUNCHES):
@@ -527,17 +537,13 @@ def main():
     f0 = tgv_populations(L, q, n)
     nodes = n * n
 
-    print(f"[bench] FP64: {W} warmup + {K} timed steps", file=sys.stderr, flush=True)
-    m64, wall64, kms64, nl64, clk64 = run_storage(L, q, f0, cm, "f64", K, W)
+    acc_steps = a.accuracy_steps
+    m64, wall64, kms64, nl64, clk64, u64 = run_storage(L, q, f0, cm, "f64", K, W, acc_steps)
     mlups64 = nodes * K / wall64 / 1e6
     bytes64 = 2 * q.Q * 8 * nodes
     kavg64 = kms64 / max(nl64, 1)
     gbs64 = bytes64 / (kavg64 * 1e-3) / 1e9
     spread64 = launch_spread(L, m64)
-    acc_steps = a.accuracy_steps
-    print(f"[bench] FP64 accuracy reference: {acc_steps} untimed steps", file=sys.stderr,
-          flush=True)
-    u64 = velocity_after(L, q, f0, cm, "f64", acc_steps)
 
     result = {
         "metric": METRIC,
@@ -582,10 +588,10 @@ def main():
         # record (*neq: 2 Q + 8 bytes per node), rest-state deviations (*dev),
         # and shifted FP32
         kernels = {
-            "f16neq": "k_fused_neq_rows<9,TRT,F16Neq> (8 rows/thread, LDS record window, "
-                      "edge nodes in its last block column)",
-            "bf16neq": "k_fused_neq_rows<9,TRT,BF16Neq> (8 rows/thread, LDS record window, "
-                       "edge nodes in its last block column)",
+            "f16neq": "k_fused_neq_rows2<9,TRT,F16Neq> (two columns x 2 rows per lane, LDS "
+                      "record window, edge nodes in its last block column)",
+            "bf16neq": "k_fused_neq_rows2<9,TRT,BF16Neq> (two columns x 2 rows per lane, LDS "
+                       "record window, edge nodes in its last block column)",
             "f16dev": "k_fused_v<9,TRT,F16Dev> (4 nodes/thread)",
             "bf16dev": "k_fused_v<9,TRT,BF16Dev> (4 nodes/thread)",
             "f32": "k_fused_v<9,TRT,F32> (256x1 blocks, 1 node/thread)",
@@ -595,18 +601,14 @@ def main():
                                        ("bf16neq", "bf16neq", 2 * q.Q + 8),
                                        ("bf16dev", "bf16dev", 2 * q.Q),
                                        ("f32", "fp32", 4 * q.Q)):
-            print(f"[bench] {storage}: {W} warmup + {K} timed steps", file=sys.stderr,
-                  flush=True)
-            m16, wall16, kms16, nl16, clk16 = run_storage(L, q, f0, cm, storage, K, W)
+            m16, wall16, kms16, nl16, clk16, u16 = run_storage(L, q, f0, cm, storage, K, W,
+                                                               acc_steps)
             mlups16 = nodes * K / wall16 / 1e6
             bytes16 = 2 * per_node * nodes
             kavg16 = kms16 / max(nl16, 1)
             gbs16 = bytes16 / (kavg16 * 1e-3) / 1e9
             spread16 = launch_spread(L, m16)
             del m16
-            print(f"[bench] {storage} accuracy: {acc_steps} untimed steps", file=sys.stderr,
-                  flush=True)
-            u16 = velocity_after(L, q, f0, cm, storage, acc_steps)
             result[key] = {
                 "value": round(mlups16, 1),
                 "unit": "MLUPS",

@@ -304,8 +304,8 @@ int max_vec(int Q, int storage) {
 // 4096^2, D2Q17 / D2Q37 8192 x 2048): non-temporal stores are fastest for
 // every storage; one node per thread for 8- and 4-byte populations and for
 // 16-bit deviations on the wide stencils, four for 16-bit deviations on
-// D2Q9 (one dwordx2 per plane per lane); for the non-equilibrium storages 8
-// rows per thread (k_fused_neq_rows, records in an LDS window).
+// D2Q9 (one dwordx2 per plane per lane); for the non-equilibrium storages
+// the row kernels (records in an LDS window) with the rows per thread below.
 // lbm_autotune refines this per lattice.
 void auto_launch(const lbm_model_s *m, const lbm_field_s *f, int &block, int &rows, int &vec,
                  int &nt) {
@@ -330,10 +330,15 @@ void auto_launch(const lbm_model_s *m, const lbm_field_s *f, int &block, int &ro
     if (block * rows > 256) rows = 256 / block;
     vec = m->launch_vec;
     nt = m->launch_nt;
-    // non-equilibrium storages: 8 rows per thread, 2 on the 21- and
-    // 37-velocity stencils (D2Q37 FP16-NEQ 8192 x 2048: 1019 us against
-    // 1131 us with 8; D2Q9 4096^2: 197-199 us with 4 or 8, 215 us with 2)
-    if (vec <= 0 && rows_mode(f->storage) && m->coll != LBM_MRT) vec = f->Q > 17 ? 2 : 8;
+    // non-equilibrium storages, rows per thread (MI355X, FP16-NEQ):
+    // * D2Q4/5/9 on even widths run two columns per lane: 2 rows (D2Q9 4096^2,
+    //   256 threads: 166-169 us against 181 with 4 and 189 with 8);
+    // * D2Q4/5/9 on odd widths (one column per lane): 4 rows (4095^2: 224 us
+    //   against 227 with 2 and 245 with 8);
+    // * D2Q13/17: 8 rows (D2Q17 8192 x 2048: 382-387 us against 418-420 with 4);
+    // * D2Q21/37: 2 rows (D2Q37 8192 x 2048: 887-892 us against 1015-1035 with 4).
+    if (vec <= 0 && rows_mode(f->storage) && m->coll != LBM_MRT)
+        vec = f->Q > 17 ? 2 : (f->Q > 9 ? 8 : (f->nx % 2 == 0 && f->nx > 4 ? 2 : 4));
     if (vec <= 0) {
         vec = 1;
         const bool neq = f->storage == LBM_STORE_F16NEQ || f->storage == LBM_STORE_BF16NEQ;

@@ -525,13 +525,18 @@ def test_reduced_storage_taylor_green(storage):
 
 
 @pytest.mark.parametrize("storage", ["f16neq", "bf16neq"])
-@pytest.mark.parametrize("rows", [2, 8, 16])
-@pytest.mark.parametrize("shape", [(3, 3), (4, 5), (1, 6), (7, 1), (2, 9), (33, 19), (64, 21)])
+@pytest.mark.parametrize("rows", [2, 4, 8, 16])
+@pytest.mark.parametrize("shape", [(3, 3), (4, 5), (1, 6), (7, 1), (2, 9), (6, 5), (10, 3),
+                                   (33, 19), (64, 21), (130, 7), (514, 9)])
 @pytest.mark.parametrize("bcname", ["periodic", "couette"])
 def test_neq_row_marching_matches_single_node(storage, rows, shape, bcname):
     """The row-marching kernel (interior) + edge kernel against the one node
     per thread kernel, for lattices with fewer rows than a thread marches,
     one-node-wide lattices and heights that are not multiples of the row count.
+    Even widths above 4 run the two-columns-per-lane kernel: 6 is the
+    narrowest (one interior pair per row), and 130 / 514 leave a single pair
+    in the last block of 64 / 256 threads; odd widths run one column per
+    lane.
     The two evaluate the same expressions in a different instruction order, so
     they agree bit for bit until an FP32 rounding difference flips one 16-bit
     rounding; from then on they differ by 16-bit ulps of the non-equilibrium
