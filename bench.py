@@ -375,10 +375,14 @@ def mrt_leg(L, steps=20):
                                        -0.02 * np.cos(X) * np.sin(Y))
         del X, Y
         N = max(2, L.order(q) // 2)
-        for name, cm in (("MRT", L.MRT(q, [0.8] * N)), ("SRT", L.SRT(0.8))):
+        # "MRT-MFMA": the same collision on the matrix cores (16-node tiles)
+        for name, cm in (("MRT", L.MRT(q, [0.8] * N)), ("MRT-MFMA", L.MRT(q, [0.8] * N)),
+                         ("SRT", L.SRT(0.8))):
             print(f"[bench] D2Q{qid} {nx}x{ny} FP64 {name}: {steps} steps", file=sys.stderr,
                   flush=True)
             m = L.LatticeBoltzmannModel.from_populations(q, f0, cm)
+            if name == "MRT-MFMA":
+                m.launch = (256, 1, 16, 1)
             L.simulate(m, range(3))
             ms, nl = L.simulate_timed(m, steps)
             us = ms / max(nl, 1) * 1e3

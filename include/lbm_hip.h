@@ -170,8 +170,13 @@ int lbm_model_set_bcs(lbm_model m, const lbm_bc *bcs, int n);
  * nodes per thread (1, 2, 4, 8; along x, except for the non-equilibrium
  * storages, where more than one are rows marched by one thread: 2..16) and
  * non-temporal stores (0/1);
- * 0 (or -1 for nontemporal) = automatic.  No counterpart in the reference (a
- * CPU loop has no launch shape). */
+ * 0 (or -1 for nontemporal) = automatic.  For MRT (FP64), nodes_per_thread =
+ * 16 selects the matrix-core kernel (tiles of 16 nodes, FP64 MFMA over the
+ * distinct Hermite components; periodic lattices with nx a multiple of 16,
+ * other cases return an error; parity with the oracle to 1e-12 relative
+ * instead of bit for bit).  It is opt-in: on MI355X it is slower than the
+ * default vector kernel.  No counterpart in the reference (a CPU loop has no
+ * launch shape). */
 int lbm_model_set_launch(lbm_model m, int block_x, int rows_per_block, int nodes_per_thread,
                          int nontemporal);
 /* Autotuner: times every launch shape of the fused step for this model on

@@ -810,6 +810,7 @@ __device__ __forceinline__ void collide_mrt_node(T (&f)[QT<QID>::Q], const StepP
     mrt_each(out, std::make_integer_sequence<int, QT<QID>::Q>{});
 }
 
+
 // ---------------------------------------------------------------------------
 // boundary conditions of the ordered list for node (x, y), applied to the
 // pulled populations f (bounce_back.jl:8-72, moving_wall.jl:17-38):
@@ -1927,6 +1928,246 @@ int launch_neq_rows_r(const Launch &L, const StepParams<typename ST::T> &p, hipS
     return hipGetLastError() == hipSuccess ? 0 : -4;
 }
 
+// ---------------------------------------------------------------------------
+// Regularised MRT on the matrix cores (FP64 MFMA), periodic lattices.
+//
+// The Hermite tensors are symmetric, so the projection and reconstruction
+// need only their distinct entries: component m = 0 (H0 = 1), 1, 2 (H1 = c)
+// and, per order n = 2..N, one entry per (a, b) = (number of x, number of y
+// indices), a + b = n -- 15 components for N = 4, 10 for N = 3, 6 for N = 2,
+// padded to 16.  Per tile of 16 nodes:
+//   a_f[m][node] = sum_i H_m(c_i) f_i[node]       (16 x Qp) x (Qp x 16)
+//   f_i[node]    = sum_m W[i][m] a_coll[m][node]  (Qr x 16) x (16 x 16)
+// with W[i][m] = w_i {1, s c_x, s c_y, s^n / n! binom(n, a) H_(a,b)(c_i)}
+// (the binomial counts the equal entries of the full tensor) and a_coll the
+// relaxed coefficients of mrt.jl:99-104 (order 0 and 1: rho and rho u with the
+// force-shifted u, as in mrt.jl:106-115).  v_mfma_f64_16x16x4f64 takes A
+// (16 x 4) with lane l holding A[l % 16][l / 16], B (4 x 16) with lane l
+// holding B[l / 16][l % 16], and returns D (16 x 16) with lane l, register r
+// holding D[l / 16 + 4 r][l % 16] (checked on gfx950 with a probe of known
+// operands).  So lane l = 16 j + n pulls the
+// populations i = j, j + 4, j + 8, ... of node n -- the B operand of the
+// projection's K-steps as loaded -- and the A operands (coefficients) sit in
+// LDS in lane order.  The projection's D goes through a per-wave LDS buffer
+// to become the reconstruction's B.  Sums run in the matrix core's order:
+// parity with the oracle is to rounding (1e-12 relative over a run), not bit
+// for bit.
+// ---------------------------------------------------------------------------
+template <int QID>
+struct MrtTile {
+    static constexpr int Q = QT<QID>::Q;
+    static constexpr int N = QT<QID>::ORDER / 2;
+    static constexpr int KP = (Q + 3) / 4;    // projection K-steps (4 populations each)
+    static constexpr int CR = (Q + 15) / 16;  // reconstruction row blocks (16 populations)
+    static constexpr int NC = 3 + (N >= 2 ? 3 : 0) + (N >= 3 ? 4 : 0) + (N >= 4 ? 5 : 0);
+    static_assert(NC <= 16, "distinct Hermite components exceed one 16-wide block");
+};
+// component m -> (order n, number a of x indices); m = 0: (0, 0), 1: (1, 1), 2: (1, 0)
+LBM_HD constexpr void mrt_comp(int m, int &n, int &a) {
+    if (m == 0) { n = 0; a = 0; return; }
+    if (m <= 2) { n = 1; a = m == 1 ? 1 : 0; return; }
+    int base = 3;
+    for (int o = 2; o <= 4; ++o) {
+        if (m < base + o + 1) { n = o; a = o - (m - base); return; }
+        base += o + 1;
+    }
+    n = -1; a = 0;
+}
+LBM_HD constexpr int binom_small(int n, int k) {
+    int r = 1;
+    for (int i = 0; i < k; ++i) r = r * (n - i) / (i + 1);
+    return r;
+}
+// H_(a,b)(c_i) of order n through the full-tensor entry whose first a indices
+// are x (bit 0) and the rest y (bit 1)
+template <int QID>
+__device__ double mrt_herm_distinct(int n, int a, int i, double cs, double cs2) {
+    if (n == 0) return 1.0;
+    if (n == 1) return (double)(a == 1 ? QT<QID>::cx(i) : QT<QID>::cy(i));
+    const int k = ((1 << (n - a)) - 1) << a;
+    const HermInt h = herm_int<QID>(n, i, k);
+    return ((double)h.A - cs * (double)h.B) + cs2 * (double)h.C;
+}
+
+template <int QID, bool NT>
+__global__ void __launch_bounds__(256)
+k_mrt_mfma(const double *__restrict__ C, double *__restrict__ Cn, const StepParams<double> p,
+           int iters) {
+    using TT = MrtTile<QID>;
+    constexpr int Q = TT::Q, KP = TT::KP, CR = TT::CR, NC = TT::NC;
+    using d4 = double __attribute__((ext_vector_type(4)));
+    __shared__ double tabP[KP][64];       // projection A operands
+    __shared__ double tabR[CR][4][64];    // reconstruction A operands
+    __shared__ double xch[4][16][17];     // per wave: a_f of its 16 nodes
+    const double cs = p.cs_inv, cs2 = cs * cs;
+    for (int e = threadIdx.x; e < KP * 64; e += blockDim.x) {
+        const int kk = e / 64, l = e % 64, m = l % 16, i = 4 * kk + l / 16;
+        int n, a;
+        mrt_comp(m, n, a);
+        tabP[kk][l] = (m < NC && i < Q) ? mrt_herm_distinct<QID>(n, a, i, cs, cs2) : 0.0;
+    }
+    for (int e = threadIdx.x; e < CR * 4 * 64; e += blockDim.x) {
+        const int c = e / 256, kk = (e / 64) % 4, l = e % 64;
+        const int i = 16 * c + l % 16, m = 4 * kk + l / 16;
+        double v = 0.0;
+        if (m < NC && i < Q) {
+            int n, a;
+            mrt_comp(m, n, a);
+            double fac = 1.0;
+            if (n >= 1) fac = p.s;
+            if (n >= 2) fac = p.mrt_sn[n] * (double)binom_small(n, a) / (double)(n == 2 ? 2 : (n == 3 ? 6 : 24));
+            v = p.w[i] * fac * mrt_herm_distinct<QID>(n, a, i, cs, cs2);
+        }
+        tabR[c][kk][l] = v;
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int j = lane >> 4, nd = lane & 15;
+    // this lane's reconstruction components m = 4 kk + j, packed (order + 1)
+    // | (x count << 4) per byte; order 0 = padding
+    unsigned tags = 0;
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk) {
+        const int m = 4 * kk + j;
+        int n = -1, a = 0;
+        if (m < NC) mrt_comp(m, n, a);
+        tags |= (unsigned)((n + 1) | (a << 4)) << (8 * kk);
+    }
+    // the pulled populations i = j + 4 t: byte offset of the source from the
+    // node within plane i (interior tiles)
+    int soff[KP];
+#pragma unroll
+    for (int t = 0; t < KP; ++t) {
+        const int i = j + 4 * t < Q ? j + 4 * t : 0;
+        soff[t] = 8 * (-QT<QID>::cy(i) * p.nx - QT<QID>::cx(i));
+    }
+    constexpr int M = QT<QID>::MAXC;
+    const int tiles_x = p.nx / 16;
+    // wave-uniform tile coordinates, stepped by the grid's stride each iteration
+    const int stride = (int)gridDim.x * 4;
+    const int t0 = (int)blockIdx.x * 4 + wv;
+    int ty = t0 / tiles_x, tx = t0 - (t0 / tiles_x) * tiles_x;
+    const int sty = stride / tiles_x, stx = stride - sty * tiles_x;
+    // lane part of the byte offsets: plane j plus the node (loads and stores)
+    const unsigned ld_lane = 8u * (unsigned)(j * p.plane);
+    double bf[KP];
+    // the populations of tile (tx, ty) into bf
+    auto pull = [&](int ptx, int pty) {
+        const bool plive = pty < p.ny;
+        const int x = ptx * 16 + nd, y = plive ? pty : 0;
+        const unsigned node8 = 8u * (unsigned)(p.nx * y + x);
+        const bool inner = ptx * 16 >= M && ptx * 16 + 16 <= p.nx - M && y >= M && y < p.ny - M;
+        if (inner) {
+#pragma unroll
+            for (int t = 0; t < KP; ++t) {
+                // plane 4 t from a uniform base, plane j and the shift per lane
+                const double *base = C + (long long)(4 * t) * p.plane;
+                bf[t] = j + 4 * t < Q ? ld_g(at_b(base, ld_lane + node8 + (unsigned)soff[t])) : 0.0;
+            }
+        } else {
+#pragma unroll
+            for (int t = 0; t < KP; ++t) {
+                double v = 0.0;
+                const int i = j + 4 * t;
+                if (plive && i < Q) {
+                    int xs = x - QT<QID>::cx(i), ys = y - QT<QID>::cy(i);
+                    xs = xs < 0 ? xs + p.nx : (xs >= p.nx ? xs - p.nx : xs);
+                    ys = ys < 0 ? ys + p.ny : (ys >= p.ny ? ys - p.ny : ys);
+                    v = C[p.plane * i + (long long)p.nx * ys + xs];
+                }
+                bf[t] = v;
+            }
+        }
+    };
+    pull(tx, ty);
+    for (int it = 0; it < iters; ++it) {
+        const bool live = ty < p.ny;
+        const int x = tx * 16 + nd, y = live ? ty : 0;
+        const unsigned node8 = 8u * (unsigned)(p.nx * y + x);
+        d4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int t = 0; t < KP; ++t)
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(tabP[t][lane], bf[t], acc, 0, 0, 0);
+        // the next tile's populations load while this one relaxes and
+        // reconstructs (into the registers the projection has consumed)
+        tx += stx;
+        ty += sty;
+        if (tx >= tiles_x) {
+            tx -= tiles_x;
+            ++ty;
+        }
+        if (it + 1 < iters) pull(tx, ty);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) xch[wv][nd][j + 4 * r] = acc[r];
+        __builtin_amdgcn_wave_barrier();
+        const double rho = xch[wv][nd][0];
+        double ux = xch[wv][nd][1] / rho, uy = xch[wv][nd][2] / rho;
+        if (p.force_kind != 0 && live) {
+            double Fx, Fy;
+            node_force(p, x, y, Fx, Fy);
+            ux = ux + p.tau_force * Fx;  // mrt.jl:94
+            uy = uy + p.tau_force * Fy;
+        }
+        double bc[4];
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) {
+            const unsigned tg = (tags >> (8 * kk)) & 0xffu;
+            const int n = (int)(tg & 15u) - 1, a = (int)(tg >> 4);
+            double v;
+            if (n == 0) v = rho;
+            else if (n == 1) v = rho * (a == 1 ? ux : uy);
+            else if (n < 0) v = 0.0;
+            else {
+                double eq = rho;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) eq = eq * (q < a ? ux : (q < n ? uy : 1.0));
+                const double omo = n == 2 ? p.mrt_omo[2] : (n == 3 ? p.mrt_omo[3] : p.mrt_omo[4]);
+                const double om = n == 2 ? p.mrt_om[2] : (n == 3 ? p.mrt_om[3] : p.mrt_om[4]);
+                v = omo * xch[wv][nd][4 * kk + j] + om * eq;  // mrt.jl:99-104
+            }
+            bc[kk] = v;
+        }
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int c = 0; c < CR; ++c) {
+            d4 o = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int kk = 0; kk < 4; ++kk)
+                o = __builtin_amdgcn_mfma_f64_16x16x4f64(tabR[c][kk][lane], bc[kk], o, 0, 0, 0);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int i = 16 * c + 4 * r + j;
+                if (live && i < Q) {
+                    // plane 16 c + 4 r from a uniform base, plane j and the node per lane
+                    double *base = Cn + (long long)(16 * c + 4 * r) * p.plane;
+                    gptr<double> dst = at_b(base, ld_lane + node8);
+                    if constexpr (NT) __builtin_nontemporal_store(o[r], dst);
+                    else *dst = o[r];
+                }
+            }
+        }
+    }
+}
+
+// 16 tiles of 16 nodes per wave, four waves per block; periodic lattices
+// (no boundary list) with nx a multiple of 16
+template <int QID>
+int launch_mrt_mfma(const void *src, void *dst, bool nt, const StepParams<double> &p,
+                    hipStream_t s) {
+    if (p.nbc != 0 || p.nx % 16 != 0) return -5;
+    // 32-bit byte offsets: plane j <= 3 plus the node and a shift below 4 GiB
+    if (4 * 8 * p.plane >= (1LL << 32)) return -5;
+    const long long ntiles = (long long)(p.nx / 16) * p.ny;
+    const long long blocks = (ntiles + 63) / 64;
+    const int iters = (int)((ntiles + blocks * 4 - 1) / (blocks * 4));
+    const double *C = static_cast<const double *>(src);
+    double *Cn = static_cast<double *>(dst);
+    if (nt) k_mrt_mfma<QID, true><<<(unsigned)blocks, 256, 0, s>>>(C, Cn, p, iters);
+    else k_mrt_mfma<QID, false><<<(unsigned)blocks, 256, 0, s>>>(C, Cn, p, iters);
+    return hipGetLastError() == hipSuccess ? 0 : -4;
+}
+
 template <int QID, int COLL, class ST>
 int launch_fused_c(const Launch &L, const StepParams<typename ST::T> &p, hipStream_t s) {
     constexpr int MV = max_vec<QID, ST>();
@@ -1984,8 +2225,10 @@ int launch_op(const Launch &L, const StepParams<typename ST::T> &p, hipStream_t 
         if (L.coll == 3) return -2;  // collide-only model: the host runs collide + stream
         if (L.coll == 2) {
             // MRT (FP64 only): with non-temporal stores through the row-shaped
-            // kernel, one node per thread
+            // kernel, one node per thread; 16 nodes per thread asks for the
+            // matrix-core tiles (periodic lattices)
             if constexpr (ST::id == 0) {
+                if (L.vec == 16) return launch_mrt_mfma<QID>(L.src, L.dst, L.nt != 0, p, s);
                 if (L.nt) return launch_fused_vn<QID, 2, ST, 1>(L, p, s);
                 k_fused<QID, 2, ST><<<grid, block, 0, s>>>(src, dst, p);
             } else {
