@@ -51,18 +51,21 @@ SPREAD_LAUNCHES = 200
 
 
 def run_storage(L, q, f0, cm, storage, steps, warmup, accuracy_steps):
-    """The velocity after an untimed ``accuracy_steps``-step run of this
-    storage, then, on a model uploaded before that run, W untimed warm-up
-    steps and K timed steps (wall clock around them, HIP events around the
-    fused launches), with the shader clock sampled before and after the timed
-    steps.  The accuracy run goes first so that the timed launches start from
-    the GPU's sustained state rather than from the idle stretch of a host
-    upload (clock-sensitive kernels run their first launches after idle up to
-    10% slower)."""
+    """An untimed ``accuracy_steps``-step run of this storage (its velocity is
+    returned), then, on a second model uploaded before that run, W untimed
+    warm-up steps and K timed steps (wall clock around them, HIP events around
+    the fused launches), with the shader clock sampled before and after the
+    timed steps.  The accuracy run goes first and its velocity is read back
+    only after the timed steps, so the warm-up follows it on the device with
+    no idle stretch in between: the timed launches start from the GPU's
+    sustained state, not from the idle stretch of a host upload or download
+    (clock-sensitive kernels run their first launches after idle up to 10%
+    slower)."""
     model = L.LatticeBoltzmannModel.from_populations(q, f0, cm, storage=storage)
+    acc = L.LatticeBoltzmannModel.from_populations(q, f0, cm, storage=storage)
     print(f"[bench] {storage}: accuracy run, {accuracy_steps} untimed steps", file=sys.stderr,
           flush=True)
-    u = velocity_after(L, q, f0, cm, storage, accuracy_steps)
+    L.simulate(acc, range(accuracy_steps))
     print(f"[bench] {storage}: {warmup} warmup + {steps} timed steps", file=sys.stderr,
           flush=True)
     if warmup:
@@ -74,7 +77,9 @@ def run_storage(L, q, f0, cm, storage, steps, warmup, accuracy_steps):
     L.synchronize()
     wall = time.perf_counter() - t0
     c1 = L.clock_sample()
-    return model, wall, kernel_ms, nlaunch, L.average_clock_mhz(c0, c1), u
+    m = L.moments(acc.f_stream, stress=False)
+    del acc
+    return model, wall, kernel_ms, nlaunch, L.average_clock_mhz(c0, c1), (m.ux, m.uy)
 
 
 def launch_spread(L, model, launches=SPREAD_LAUNCHES):
@@ -93,15 +98,6 @@ def launch_spread(L, model, launches=SPREAD_LAUNCHES):
 
 def _r(v, nd=1):
     return None if v is None else round(v, nd)
-
-
-def velocity_after(L, q, f0, cm, storage, steps):
-    """The velocity field after ``steps`` steps (untimed)."""
-    model = L.LatticeBoltzmannModel.from_populations(q, f0, cm, storage=storage)
-    L.simulate(model, range(steps))
-    m = L.moments(model.f_stream, stress=False)
-    del model
-    return m.ux, m.uy
 
 
 def rel_l2(a, b):
