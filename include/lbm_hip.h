@@ -203,7 +203,9 @@ int lbm_stream_bc(lbm_model m, lbm_field f, lbm_field f_new);
  * what the reference's model.f_stream / model.f_collision hold. */
 int lbm_simulate(lbm_model m, lbm_field f_stream, lbm_field f_collision, int64_t nsteps);
 /* As lbm_simulate, also returning the time of the fused kernels (HIP events on
- * the library stream): *ms_total over all nsteps, *nlaunch fused launches. */
+ * the library stream): *ms_total over the fused launches of the run, *nlaunch
+ * their number (nsteps - 1, or nsteps - 2 where the first step runs as one
+ * collide + fused pass: FP64 D2Q4/5/9 on lattices of whole 64 x 8 tiles). */
 int lbm_simulate_timed(lbm_model m, lbm_field f_stream, lbm_field f_collision, int64_t nsteps,
                        double *ms_total, int64_t *nlaunch);
 /* Measurement support; no counterpart in the reference.  As lbm_simulate,

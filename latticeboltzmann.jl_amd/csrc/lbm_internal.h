@@ -102,7 +102,11 @@ struct BatchTile {
     int n0;
 };
 
-enum class Op : int { Collide = 0, StreamBC = 1, Fused = 2, ApplyBC = 3, Copy = 4, Convert = 5 };
+// FusedFirst: the first step of a run from the canonical state, collide +
+// stream + BCs + collide in one pass (FP64, one-speed stencils; the host
+// checks the tile shape)
+enum class Op : int { Collide = 0, StreamBC = 1, Fused = 2, ApplyBC = 3, Copy = 4, Convert = 5,
+                      FusedFirst = 6 };
 
 struct Launch {
     Op op;

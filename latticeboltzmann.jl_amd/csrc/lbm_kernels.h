@@ -2361,6 +2361,85 @@ int launch_fused_v(const Launch &L, const StepParams<typename ST::T> &p, hipStre
     return L.coll == 0 ? launch_fused_c<QID, 0, ST>(L, p, s) : launch_fused_c<QID, 1, ST>(L, p, s);
 }
 
+// The first step of a run, from the canonical state S_0, in one pass: the
+// run otherwise starts with collide! over the whole lattice (C_0) and then
+// pulls from C_0 in the first fused step, two passes over HBM.  Here a block
+// collides the S_0 of its tile and a one-node halo into LDS, then each node of
+// the tile pulls from LDS, applies the boundary list (its own C_0 from LDS),
+// collides and stores C_1.  The collision and boundary code is the fused
+// kernel's, so C_1 is bit-identical to the two-pass path.  One-speed stencils
+// (halo 1), FP64; tiles of 64 x 8 nodes (the host requires nx % 64 == 0 and
+// ny % 8 == 0), two nodes per thread; the halo costs 29% more collisions and
+// reads, most of the re-read rows coming from L2.
+template <int QID, int COLL, bool NT>
+__global__ void __launch_bounds__(256)
+k_fused_first(const double *__restrict__ S, double *__restrict__ Cn, const StepParams<double> p) {
+    constexpr int Q = QT<QID>::Q;
+    static_assert(QT<QID>::MAXC == 1, "the first-step tile has a one-node halo");
+    constexpr int TX = 64, TY = 8, HX = TX + 2, HY = TY + 2, HN = HX * HY;
+    __shared__ double c0[Q][HN];
+    const int tid = threadIdx.x;
+    const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY;
+    for (int h = tid; h < HN; h += blockDim.x) {
+        const int hx = h % HX, hy = h / HX;
+        const int gx = wrap(x0 - 1 + hx, p.nx), gy = wrap(y0 - 1 + hy, p.ny);
+        double f[Q];
+        load_local<QID, StF64>(S, p, (long long)gx + (long long)p.nx * gy, f, LBM_SEQ);
+        collide_any<QID, COLL>(f, p, gx, gy);
+#pragma unroll
+        for (int i = 0; i < Q; ++i) c0[i][h] = f[i];
+    }
+    __syncthreads();
+#pragma unroll 1
+    for (int r = 0; r < TY / 4; ++r) {
+        const int tx = tid % TX, ty = tid / TX + 4 * r;
+        const int x = x0 + tx, y = y0 + ty;
+        const int h = (ty + 1) * HX + (tx + 1);
+        double f[Q];
+        auto pull = [&](auto Ic) {
+            constexpr int i = decltype(Ic)::value;
+            f[i] = c0[i][h - QT<QID>::cx(i) - HX * QT<QID>::cy(i)];
+        };
+        adj_each_node<QID>(pull);
+        // the boundary list (bounce_back.jl:8-72, moving_wall.jl:17-38) as in
+        // apply_bc_seq, with the node's own post-collision populations from LDS
+        if (p.nbc > 0 && (x == 0 || x == p.nx - 1 || y == 0 || y == p.ny - 1)) {
+            for (int b = 0; b < p.nbc; ++b) {
+                const BCDev bc = p.bcs[b];
+                const bool inr =
+                    bc.kind == 1 || (x >= bc.x0 && x <= bc.x1 && y >= bc.y0 && y <= bc.y1);
+                if (!inr) continue;
+                auto one = [&](auto Ic) {
+                    constexpr int i = decltype(Ic)::value;
+                    constexpr int cx = QT<QID>::cx(i), cy = QT<QID>::cy(i), j = QT<QID>::opp(i);
+                    bool hit;
+                    switch (bc.dir) {
+                    case 0: hit = (y - cy) >= p.ny; break;
+                    case 2: hit = (y - cy) < 0; break;
+                    case 1: hit = (x - cx) >= p.nx; break;
+                    default: hit = (x - cx) < 0; break;
+                    }
+                    if (hit) {
+                        double v = c0[j][h];
+                        if (bc.kind == 1) v = v + p.mw[b * Q + i];
+                        f[i] = v;
+                    }
+                };
+                adj_each_node<QID>(one);
+            }
+        }
+        collide_any<QID, COLL>(f, p, x, y);
+        const long long node = (long long)x + (long long)p.nx * y;
+        auto st = [&](auto Ic) {
+            constexpr int i = decltype(Ic)::value;
+            double *dst = Cn + p.plane * i + node;
+            if constexpr (NT) __builtin_nontemporal_store(f[i], dst);
+            else *dst = f[i];
+        };
+        adj_each_node<QID>(st);
+    }
+}
+
 template <int QID, class ST>
 int launch_op(const Launch &L, const StepParams<typename ST::T> &p, hipStream_t s) {
     using S = typename ST::store_t;
@@ -2403,6 +2482,27 @@ int launch_op(const Launch &L, const StepParams<typename ST::T> &p, hipStream_t 
         break;
     case Op::StreamBC:
         k_stream_bc<QID, ST><<<grid, block, 0, s>>>(src, dst, p);
+        break;
+    case Op::FusedFirst:
+        if constexpr (ST::id == 0 && QT<QID>::MAXC == 1) {
+            if (p.nx % 64 != 0 || p.ny % 8 != 0 || L.coll > 2) return -5;
+            const dim3 g(p.nx / 64, p.ny / 8, 1);
+            const double *S0 = static_cast<const double *>(L.src);
+            double *C1 = static_cast<double *>(L.dst);
+            const bool nt = L.nt != 0;
+            if (L.coll == 0) {
+                if (nt) k_fused_first<QID, 0, true><<<g, 256, 0, s>>>(S0, C1, p);
+                else k_fused_first<QID, 0, false><<<g, 256, 0, s>>>(S0, C1, p);
+            } else if (L.coll == 1) {
+                if (nt) k_fused_first<QID, 1, true><<<g, 256, 0, s>>>(S0, C1, p);
+                else k_fused_first<QID, 1, false><<<g, 256, 0, s>>>(S0, C1, p);
+            } else {
+                if (nt) k_fused_first<QID, 2, true><<<g, 256, 0, s>>>(S0, C1, p);
+                else k_fused_first<QID, 2, false><<<g, 256, 0, s>>>(S0, C1, p);
+            }
+        } else {
+            return -5;
+        }
         break;
     case Op::ApplyBC:
         k_apply_bc<QID, ST><<<grid, block, 0, s>>>(src, dst, p);

@@ -606,3 +606,30 @@ def test_neq_row_marching_wide_stencils(qid, storage, rows, shape, bcname):
                 assert d.max() <= tol, (steps, d.max())
                 assert np.count_nonzero(d > 1e-6) <= frac * d.size, (steps,
                                                                      np.count_nonzero(d > 1e-6))
+
+
+@pytest.mark.parametrize("qid", [4, 5, 9])
+@pytest.mark.parametrize("coll", [0, 1, 2])
+@pytest.mark.parametrize("bcname,force", [("periodic", "none"), ("couette", "const"),
+                                          ("poiseuille", "const"), ("cavity", "field"),
+                                          ("partial", "none")])
+@pytest.mark.parametrize("steps", [2, 5])
+def test_first_step_tile_bit_exact(oracle_lib, qid, coll, bcname, force, steps):
+    """FP64 one-speed stencils on lattices of whole 64 x 8 tiles start a run
+    with one pass (collide S_0 into an LDS tile with a one-node halo, then
+    stream, boundaries and collide): f_stream and f_collision equal the
+    oracle's bits, with boundaries on every side and with forces."""
+    L = _L()
+    nx, ny = 128, 16
+    f0 = random_populations(qid, nx, ny, seed=53 + qid + 7 * coll)
+    F = force_for(force, nx, ny)
+    cm, taus = _cm(coll, qid, F)
+    bcs = bc_dicts(bcname, nx, ny) if bcname != "periodic" else []
+    q = _q(qid)
+    model = L.LatticeBoltzmannModel.from_populations(q, f0, cm, to_lbmhip(bcs))
+    L.simulate(model, range(steps))
+    S = f0.copy(order="F")
+    C = f0.copy(order="F")
+    oracle_lib.simulate(qid, S, C, coll, taus, steps, force=F, bcs=bcs)
+    _assert_same(model.f_stream.to_numpy(), S, f"first-step tile D2Q{qid} coll {coll} {bcname}")
+    _assert_same(model.f_collision.to_numpy(), C, f"first-step tile D2Q{qid} coll {coll} {bcname} (C)")
