@@ -1887,8 +1887,8 @@ k_fused_neq_rows2(const typename ST::store_t *__restrict__ C,
             collide_neq<QID, COLL, ST>(f[1], p, x + 1, y, recs.v[1], nq[1], LBM_SEQ);
             Pack<R2, 1> pm;
             pm.v[0] = recs;
-            const Rec16 *rrow = ST::recs(Cn) + (long long)p.nx * y;
-            put_pack<NT>(at_b(reinterpret_cast<R2 *>(const_cast<Rec16 *>(rrow)), o8), pm);
+            Rec16 *rrow = ST::recs(Cn) + (long long)p.nx * y;
+            put_pack<NT>(at_b(reinterpret_cast<R2 *>(rrow), o8), pm);
             auto st = [&](auto Ic) {
                 constexpr int i = decltype(Ic)::value;
                 Pack<P2, 1> pk;
