@@ -877,15 +877,18 @@ static int simulate_impl(lbm_model m, lbm_field S, lbm_field C, int64_t nsteps, 
         if (nlaunch) *nlaunch = 2 * nsteps;
         return 0;
     }
-    // step 1 starts from the canonical state S.  FP64 one-speed stencils on
-    // lattices of whole 64 x 8 tiles take C_1 = collide(BC(stream(collide(S_0))))
-    // in one pass (k_fused_first: one pass over HBM instead of collide! plus
-    // the first fused step); otherwise C_0 = collide(S_0) first.  Runs that
+    // step 1 starts from the canonical state S.  One-speed stencils on
+    // lattices of whole 64 x 8 tiles (FP64, and the non-equilibrium storages)
+    // take C_1 = collide(BC(stream(collide(S_0)))) in one pass (k_fused_first*:
+    // one pass over HBM instead of collide! plus the first fused step);
+    // otherwise C_0 = collide(S_0) first.  Runs that
     // time every launch keep the two-pass start, so that their launches are
     // all of one kernel.
-    const bool first_fused = nsteps >= 2 && !per_launch && S->storage == LBM_STORE_F64 &&
-                             m->coll <= LBM_MRT && (m->qid == 4 || m->qid == 5 || m->qid == 9) &&
-                             S->nx % 64 == 0 && S->ny % 8 == 0;
+    const bool first_fused =
+        nsteps >= 2 && !per_launch && (m->qid == 4 || m->qid == 5 || m->qid == 9) &&
+        S->nx % 64 == 0 && S->ny % 8 == 0 &&
+        ((S->storage == LBM_STORE_F64 && m->coll <= LBM_MRT) ||
+         (rows_mode(S->storage) && m->coll <= LBM_TRT));
     if (first_fused) {
         if (int rc = run(lbm::Op::FusedFirst, m, S, C, true)) return rc;
     } else {

@@ -2440,6 +2440,95 @@ k_fused_first(const double *__restrict__ S, double *__restrict__ Cn, const StepP
     }
 }
 
+// The same one-pass first step for the non-equilibrium storages (one-speed
+// stencils): the halo tile's C_0 is what collide! stores -- each node's S_0
+// decoded, collided and re-encoded (an 8-byte record and Q 16-bit parts) --
+// kept in LDS; each node of the tile then decodes its pulls from it, applies
+// the boundary list with its own decoded C_0, and collides and encodes as the
+// fused NEQ kernels do.  Its FP32 operations are those of collide! and of the
+// fused kernels; like any two NEQ kernels, the two-pass start and this one
+// agree to the rounding of single 16-bit values.
+template <int QID, int COLL, class ST, bool NT>
+__global__ void __launch_bounds__(256)
+k_fused_first_neq(const typename ST::store_t *__restrict__ S, typename ST::store_t *__restrict__ Cn,
+                  const StepParams<float> p) {
+    constexpr int Q = QT<QID>::Q;
+    static_assert(QT<QID>::MAXC == 1 && ST::kNeq, "one-speed stencils, non-equilibrium storages");
+    using St = typename ST::store_t;
+    constexpr int TX = 64, TY = 8, HX = TX + 2, HY = TY + 2, HN = HX * HY;
+    __shared__ Rec16 rc0[HN];
+    __shared__ St nc0[Q][HN];
+    const int tid = threadIdx.x;
+    const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY;
+    for (int h = tid; h < HN; h += blockDim.x) {
+        const int hx = h % HX, hy = h / HX;
+        const int gx = wrap(x0 - 1 + hx, p.nx), gy = wrap(y0 - 1 + hy, p.ny);
+        float f[Q];
+        load_local<QID, ST>(S, p, (long long)gx + (long long)p.nx * gy, f, LBM_SEQ);
+        collide_any<QID, COLL>(f, p, gx, gy);
+        Rec16 m;
+        float g[Q];
+        neq_split<QID, ST>(f, p, m, g);
+        rc0[h] = m;
+#pragma unroll
+        for (int i = 0; i < Q; ++i) nc0[i][h] = ST::from_f(g[i], p);
+    }
+    __syncthreads();
+#pragma unroll 1
+    for (int r = 0; r < TY / 4; ++r) {
+        const int tx = tid % TX, ty = tid / TX + 4 * r;
+        const int x = x0 + tx, y = y0 + ty;
+        const int h = (ty + 1) * HX + (tx + 1);
+        float f[Q];
+        auto pull = [&](auto Ic) {
+            constexpr int i = decltype(Ic)::value;
+            const int hs = h - QT<QID>::cx(i) - HX * QT<QID>::cy(i);
+            f[i] = geq_rec<QID, i>(ST::unpack(rc0[hs]), p) + ST::to_f(nc0[i][hs], p);
+        };
+        adj_each_node<QID>(pull);
+        // the boundary list as in apply_bc_seq, with the node's own decoded C_0
+        if (p.nbc > 0 && (x == 0 || x == p.nx - 1 || y == 0 || y == p.ny - 1)) {
+            for (int b = 0; b < p.nbc; ++b) {
+                const BCDev bc = p.bcs[b];
+                const bool inr =
+                    bc.kind == 1 || (x >= bc.x0 && x <= bc.x1 && y >= bc.y0 && y <= bc.y1);
+                if (!inr) continue;
+                auto one = [&](auto Ic) {
+                    constexpr int i = decltype(Ic)::value;
+                    constexpr int cx = QT<QID>::cx(i), cy = QT<QID>::cy(i), j = QT<QID>::opp(i);
+                    bool hit;
+                    switch (bc.dir) {
+                    case 0: hit = (y - cy) >= p.ny; break;
+                    case 2: hit = (y - cy) < 0; break;
+                    case 1: hit = (x - cx) >= p.nx; break;
+                    default: hit = (x - cx) < 0; break;
+                    }
+                    if (hit) {
+                        float v = geq_rec<QID, j>(ST::unpack(rc0[h]), p) + ST::to_f(nc0[j][h], p);
+                        if (bc.kind == 1) v = v + static_cast<float>(p.mw[b * Q + i]);
+                        f[i] = v;
+                    }
+                };
+                adj_each_node<QID>(one);
+            }
+        }
+        Rec16 rec;
+        float nq[Q];
+        collide_neq<QID, COLL, ST>(f, p, x, y, rec, nq, LBM_SEQ);
+        const long long node = (long long)x + (long long)p.nx * y;
+        Pack<Rec16, 1> pm;
+        pm.v[0] = rec;
+        put_pack<NT>(ST::recs(Cn) + node, pm);
+        auto st = [&](auto Ic) {
+            constexpr int i = decltype(Ic)::value;
+            Pack<St, 1> pk;
+            pk.v[0] = ST::raw(nq[i]);
+            put_pack<NT>(ST::pops(Cn, p.plane) + p.plane * i + node, pk);
+        };
+        adj_each_node<QID>(st);
+    }
+}
+
 template <int QID, class ST>
 int launch_op(const Launch &L, const StepParams<typename ST::T> &p, hipStream_t s) {
     using S = typename ST::store_t;
@@ -2499,6 +2588,17 @@ int launch_op(const Launch &L, const StepParams<typename ST::T> &p, hipStream_t 
             } else {
                 if (nt) k_fused_first<QID, 2, true><<<g, 256, 0, s>>>(S0, C1, p);
                 else k_fused_first<QID, 2, false><<<g, 256, 0, s>>>(S0, C1, p);
+            }
+        } else if constexpr (ST::kNeq && QT<QID>::MAXC == 1) {
+            if (p.nx % 64 != 0 || p.ny % 8 != 0 || L.coll > 1) return -5;
+            const dim3 g(p.nx / 64, p.ny / 8, 1);
+            const bool nt = L.nt != 0;
+            if (L.coll == 0) {
+                if (nt) k_fused_first_neq<QID, 0, ST, true><<<g, 256, 0, s>>>(src, dst, p);
+                else k_fused_first_neq<QID, 0, ST, false><<<g, 256, 0, s>>>(src, dst, p);
+            } else {
+                if (nt) k_fused_first_neq<QID, 1, ST, true><<<g, 256, 0, s>>>(src, dst, p);
+                else k_fused_first_neq<QID, 1, ST, false><<<g, 256, 0, s>>>(src, dst, p);
             }
         } else {
             return -5;
