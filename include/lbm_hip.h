@@ -205,8 +205,7 @@ int lbm_simulate(lbm_model m, lbm_field f_stream, lbm_field f_collision, int64_t
 /* As lbm_simulate, also returning the time of the fused kernels (HIP events on
  * the library stream): *ms_total over the fused launches of the run, *nlaunch
  * their number (nsteps - 1, or nsteps - 2 where the first step runs as one
- * collide + fused pass: D2Q4/5/9 in FP64 or the non-equilibrium storages on
- * lattices of whole 64 x 8 tiles). */
+ * collide + fused pass: D2Q4/5/9 on lattices of whole 64 x 8 tiles). */
 int lbm_simulate_timed(lbm_model m, lbm_field f_stream, lbm_field f_collision, int64_t nsteps,
                        double *ms_total, int64_t *nlaunch);
 /* Measurement support; no counterpart in the reference.  As lbm_simulate,

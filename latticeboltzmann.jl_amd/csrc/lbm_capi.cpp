@@ -878,8 +878,8 @@ static int simulate_impl(lbm_model m, lbm_field S, lbm_field C, int64_t nsteps, 
         return 0;
     }
     // step 1 starts from the canonical state S.  One-speed stencils on
-    // lattices of whole 64 x 8 tiles (FP64, and the non-equilibrium storages)
-    // take C_1 = collide(BC(stream(collide(S_0)))) in one pass (k_fused_first*:
+    // lattices of whole 64 x 8 tiles (every storage) take
+    // C_1 = collide(BC(stream(collide(S_0)))) in one pass (k_fused_first*:
     // one pass over HBM instead of collide! plus the first fused step);
     // otherwise C_0 = collide(S_0) first.  Runs that
     // time every launch keep the two-pass start, so that their launches are
@@ -887,8 +887,7 @@ static int simulate_impl(lbm_model m, lbm_field S, lbm_field C, int64_t nsteps, 
     const bool first_fused =
         nsteps >= 2 && !per_launch && (m->qid == 4 || m->qid == 5 || m->qid == 9) &&
         S->nx % 64 == 0 && S->ny % 8 == 0 &&
-        ((S->storage == LBM_STORE_F64 && m->coll <= LBM_MRT) ||
-         (rows_mode(S->storage) && m->coll <= LBM_TRT));
+        (m->coll <= LBM_TRT || (S->storage == LBM_STORE_F64 && m->coll == LBM_MRT));
     if (first_fused) {
         if (int rc = run(lbm::Op::FusedFirst, m, S, C, true)) return rc;
     } else {

@@ -2364,30 +2364,36 @@ int launch_fused_v(const Launch &L, const StepParams<typename ST::T> &p, hipStre
 // The first step of a run, from the canonical state S_0, in one pass: the
 // run otherwise starts with collide! over the whole lattice (C_0) and then
 // pulls from C_0 in the first fused step, two passes over HBM.  Here a block
-// collides the S_0 of its tile and a one-node halo into LDS, then each node of
-// the tile pulls from LDS, applies the boundary list (its own C_0 from LDS),
-// collides and stores C_1.  The collision and boundary code is the fused
-// kernel's, so C_1 is bit-identical to the two-pass path.  One-speed stencils
-// (halo 1), FP64; tiles of 64 x 8 nodes (the host requires nx % 64 == 0 and
+// collides the S_0 of its tile and a one-node halo into LDS, kept encoded as
+// collide! would store it, then each node of the tile decodes its pulls from
+// LDS, applies the boundary list (its own C_0 from LDS), collides and stores
+// C_1.  The collision and boundary code is the fused kernel's, so in FP64 C_1
+// is bit-identical to the two-pass path (the FP32-arithmetic storages agree to
+// the rounding of their FP32 operations, which the two paths order
+// differently).  One-speed stencils (halo 1), storages with one value per
+// population; tiles of 64 x 8 nodes (the host requires nx % 64 == 0 and
 // ny % 8 == 0), two nodes per thread; the halo costs 29% more collisions and
 // reads, most of the re-read rows coming from L2.
-template <int QID, int COLL, bool NT>
+template <int QID, int COLL, class ST, bool NT>
 __global__ void __launch_bounds__(256)
-k_fused_first(const double *__restrict__ S, double *__restrict__ Cn, const StepParams<double> p) {
+k_fused_first(const typename ST::store_t *__restrict__ S, typename ST::store_t *__restrict__ Cn,
+              const StepParams<typename ST::T> p) {
     constexpr int Q = QT<QID>::Q;
-    static_assert(QT<QID>::MAXC == 1, "the first-step tile has a one-node halo");
+    static_assert(QT<QID>::MAXC == 1 && !ST::kNeq, "one-speed stencils, one value per population");
+    using T = typename ST::T;
+    using St = typename ST::store_t;
     constexpr int TX = 64, TY = 8, HX = TX + 2, HY = TY + 2, HN = HX * HY;
-    __shared__ double c0[Q][HN];
+    __shared__ St c0[Q][HN];
     const int tid = threadIdx.x;
     const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY;
     for (int h = tid; h < HN; h += blockDim.x) {
         const int hx = h % HX, hy = h / HX;
         const int gx = wrap(x0 - 1 + hx, p.nx), gy = wrap(y0 - 1 + hy, p.ny);
-        double f[Q];
-        load_local<QID, StF64>(S, p, (long long)gx + (long long)p.nx * gy, f, LBM_SEQ);
+        T f[Q];
+        load_local<QID, ST>(S, p, (long long)gx + (long long)p.nx * gy, f, LBM_SEQ);
         collide_any<QID, COLL>(f, p, gx, gy);
 #pragma unroll
-        for (int i = 0; i < Q; ++i) c0[i][h] = f[i];
+        for (int i = 0; i < Q; ++i) c0[i][h] = ST::encode(f[i], i, p);
     }
     __syncthreads();
 #pragma unroll 1
@@ -2395,10 +2401,10 @@ k_fused_first(const double *__restrict__ S, double *__restrict__ Cn, const StepP
         const int tx = tid % TX, ty = tid / TX + 4 * r;
         const int x = x0 + tx, y = y0 + ty;
         const int h = (ty + 1) * HX + (tx + 1);
-        double f[Q];
+        T f[Q];
         auto pull = [&](auto Ic) {
             constexpr int i = decltype(Ic)::value;
-            f[i] = c0[i][h - QT<QID>::cx(i) - HX * QT<QID>::cy(i)];
+            f[i] = ST::decode(c0[i][h - QT<QID>::cx(i) - HX * QT<QID>::cy(i)], i, p);
         };
         adj_each_node<QID>(pull);
         // the boundary list (bounce_back.jl:8-72, moving_wall.jl:17-38) as in
@@ -2420,8 +2426,8 @@ k_fused_first(const double *__restrict__ S, double *__restrict__ Cn, const StepP
                     default: hit = (x - cx) < 0; break;
                     }
                     if (hit) {
-                        double v = c0[j][h];
-                        if (bc.kind == 1) v = v + p.mw[b * Q + i];
+                        T v = ST::decode(c0[j][h], j, p);
+                        if (bc.kind == 1) v = v + static_cast<T>(p.mw[b * Q + i]);
                         f[i] = v;
                     }
                 };
@@ -2432,9 +2438,9 @@ k_fused_first(const double *__restrict__ S, double *__restrict__ Cn, const StepP
         const long long node = (long long)x + (long long)p.nx * y;
         auto st = [&](auto Ic) {
             constexpr int i = decltype(Ic)::value;
-            double *dst = Cn + p.plane * i + node;
-            if constexpr (NT) __builtin_nontemporal_store(f[i], dst);
-            else *dst = f[i];
+            Pack<St, 1> pk;
+            pk.v[0] = ST::encode(f[i], i, p);
+            put_pack<NT>(Cn + p.plane * i + node, pk);
         };
         adj_each_node<QID>(st);
     }
@@ -2573,21 +2579,20 @@ int launch_op(const Launch &L, const StepParams<typename ST::T> &p, hipStream_t 
         k_stream_bc<QID, ST><<<grid, block, 0, s>>>(src, dst, p);
         break;
     case Op::FusedFirst:
-        if constexpr (ST::id == 0 && QT<QID>::MAXC == 1) {
-            if (p.nx % 64 != 0 || p.ny % 8 != 0 || L.coll > 2) return -5;
+        if constexpr (!ST::kNeq && QT<QID>::MAXC == 1) {
+            // MRT is FP64 only
+            if (p.nx % 64 != 0 || p.ny % 8 != 0 || L.coll > (ST::id == 0 ? 2 : 1)) return -5;
             const dim3 g(p.nx / 64, p.ny / 8, 1);
-            const double *S0 = static_cast<const double *>(L.src);
-            double *C1 = static_cast<double *>(L.dst);
             const bool nt = L.nt != 0;
             if (L.coll == 0) {
-                if (nt) k_fused_first<QID, 0, true><<<g, 256, 0, s>>>(S0, C1, p);
-                else k_fused_first<QID, 0, false><<<g, 256, 0, s>>>(S0, C1, p);
+                if (nt) k_fused_first<QID, 0, ST, true><<<g, 256, 0, s>>>(src, dst, p);
+                else k_fused_first<QID, 0, ST, false><<<g, 256, 0, s>>>(src, dst, p);
             } else if (L.coll == 1) {
-                if (nt) k_fused_first<QID, 1, true><<<g, 256, 0, s>>>(S0, C1, p);
-                else k_fused_first<QID, 1, false><<<g, 256, 0, s>>>(S0, C1, p);
-            } else {
-                if (nt) k_fused_first<QID, 2, true><<<g, 256, 0, s>>>(S0, C1, p);
-                else k_fused_first<QID, 2, false><<<g, 256, 0, s>>>(S0, C1, p);
+                if (nt) k_fused_first<QID, 1, ST, true><<<g, 256, 0, s>>>(src, dst, p);
+                else k_fused_first<QID, 1, ST, false><<<g, 256, 0, s>>>(src, dst, p);
+            } else if constexpr (ST::id == 0) {
+                if (nt) k_fused_first<QID, 2, ST, true><<<g, 256, 0, s>>>(src, dst, p);
+                else k_fused_first<QID, 2, ST, false><<<g, 256, 0, s>>>(src, dst, p);
             }
         } else if constexpr (ST::kNeq && QT<QID>::MAXC == 1) {
             if (p.nx % 64 != 0 || p.ny % 8 != 0 || L.coll > 1) return -5;

@@ -549,7 +549,8 @@ def test_neq_row_marching_matches_single_node(storage, rows, shape, bcname):
     f0 = random_populations(9, nx, ny, seed=31, noise=1e-4)
     cm = L.TRT.fields(0.7, 1.3, force_for("const", nx, ny))
     bcs = to_lbmhip(bc_dicts(bcname, nx, ny))
-    ulp_tol = {"f16neq": 1e-4, "bf16neq": 1e-3}[storage]  # bfloat16: 8x coarser than binary16
+    ulp_tol = {"f32": 1e-5, "f16dev": 1e-4, "bf16dev": 1e-3, "f16neq": 1e-4,
+               "bf16neq": 1e-3}[storage]  # bfloat16: 8x coarser than binary16
     for steps, tol, frac in ((1, ulp_tol, 0.1), (7, ulp_tol, 0.1)):
         out = []
         # (block, rows per block, nodes per thread, NT): one node per thread,
@@ -588,7 +589,8 @@ def test_neq_row_marching_wide_stencils(qid, storage, rows, shape, bcname):
     f0 = random_populations(qid, nx, ny, seed=37, noise=1e-4)
     cm = L.TRT.fields(0.7, 1.3, force_for("const", nx, ny))
     bcs = to_lbmhip(bc_dicts(bcname, nx, ny))
-    ulp_tol = {"f16neq": 1e-4, "bf16neq": 1e-3}[storage]
+    ulp_tol = {"f32": 1e-5, "f16dev": 1e-4, "bf16dev": 1e-3, "f16neq": 1e-4,
+               "bf16neq": 1e-3}[storage]
     w = np.asarray(q.weights, dtype=float)
     w = np.where(w > 0, w, 1.0)                 # D2Q21's four zero-weight planes stay 0
     # (on the smallest lattices the 7-step flips reach up to a quarter of the
@@ -636,20 +638,20 @@ def test_first_step_tile_bit_exact(oracle_lib, qid, coll, bcname, force, steps):
 
 
 @pytest.mark.parametrize("qid", [4, 5, 9])
-@pytest.mark.parametrize("storage", ["f16neq", "bf16neq"])
+@pytest.mark.parametrize("storage", ["f32", "f16dev", "bf16dev", "f16neq", "bf16neq"])
 @pytest.mark.parametrize("coll", [0, 1])
 @pytest.mark.parametrize("bcname,force", [("periodic", "none"), ("couette", "const"),
                                           ("cavity", "field"), ("partial", "none")])
-def test_first_step_tile_neq(oracle_lib, qid, storage, coll, bcname, force):
-    """The non-equilibrium storages on lattices of whole 64 x 8 tiles start a
-    run with one pass (the halo tile's collide! output kept in LDS).  Against
+def test_first_step_tile_reduced(oracle_lib, qid, storage, coll, bcname, force):
+    """The reduced storages on lattices of whole 64 x 8 tiles start a run with
+    one pass (the halo tile's collide! output kept in LDS as stored).  Against
     the two-pass start (collide!, then the fused step; the per-launch timing
-    entry point keeps it): equal up to the flip of single 16-bit roundings
-    (the kernels order FP32 operations differently), as for the other NEQ
-    kernel pairs; a wrong pull, halo or boundary value moves populations by
-    ~1e-2 w.  Against the oracle's FP64 run: the same velocity error as the
-    two-pass start (the storage's own error; BF16 on the D2Q4/5 sets of random
-    populations reaches 6e-4)."""
+    entry point keeps it): equal up to the rounding of the FP32 operations,
+    which the kernels order differently -- for the 16-bit storages that flips
+    single 16-bit roundings, as for the other kernel pairs; a wrong pull, halo
+    or boundary value moves populations by ~1e-2 w.  Against the oracle's
+    FP64 run: the same velocity error as the two-pass start (the storage's own
+    error; BF16 on the D2Q4/5 sets of random populations reaches 6e-4)."""
     from oracle import lbm_numpy as ln
     L = _L()
     nx, ny = 128, 16
@@ -664,7 +666,8 @@ def test_first_step_tile_neq(oracle_lib, qid, storage, coll, bcname, force):
     two = L.LatticeBoltzmannModel.from_populations(q, f0, cm, to_lbmhip(bcs), storage=storage)
     L.simulate_launch_times(two, steps)
     w = np.asarray(q.weights, dtype=float)
-    ulp_tol = {"f16neq": 1e-4, "bf16neq": 1e-3}[storage]
+    ulp_tol = {"f32": 1e-5, "f16dev": 1e-4, "bf16dev": 1e-3, "f16neq": 1e-4,
+               "bf16neq": 1e-3}[storage]
     for a, b in ((one.f_stream.to_numpy(), two.f_stream.to_numpy()),
                  (one.f_collision.to_numpy(), two.f_collision.to_numpy())):
         d = np.abs((a - b) / w)
