@@ -446,16 +446,21 @@ int lbm_synchronize(void) {
 int lbm_clock_sample(uint64_t *out, double *tick_mhz) {
     if (!out) return fail(-1, "lbm_clock_sample: null output");
     if (int rc = ensure_stream()) return rc;
-    static unsigned long long *d = nullptr;
+    // one probe buffer per device, allocated on first use with that device
+    // current and kept until the process ends
+    static unsigned long long *bufs[64] = {};
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) return fail(-3, "device index %d out of range", dev);
     constexpr size_t bytes = 3 * (size_t)LBM_CLOCK_PROBES * sizeof(unsigned long long);
-    if (!d) HIP_TRY(hipMalloc(&d, bytes));
+    if (!bufs[dev]) HIP_TRY(hipMalloc(&bufs[dev], bytes));
+    unsigned long long *d = bufs[dev];
     if (lbm::dispatch_clock_probe(d, LBM_CLOCK_PROBES, g_stream))
         return fail(-4, "clock probe launch failed");
     HIP_TRY(hipMemcpyAsync(out, d, bytes, hipMemcpyDeviceToHost, g_stream));
     HIP_TRY(hipStreamSynchronize(g_stream));
     if (tick_mhz) {
-        int dev = 0, khz = 0;
-        HIP_TRY(hipGetDevice(&dev));
+        int khz = 0;
         HIP_TRY(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
         *tick_mhz = khz / 1000.0;
     }
