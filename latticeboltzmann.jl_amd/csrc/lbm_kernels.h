@@ -378,6 +378,21 @@ __device__ __forceinline__ float geq_rec(const Mom4 &m, const StepParams<float> 
         const float cA = udot<QID, I>(m.a1, m.a2);
         if constexpr (FEQ == 2) return p.w[I] * ((m.a0 + cA) + m.a3 * (cA * cA));
         else return p.w[I] * (m.a0 + cA);
+    } else if constexpr (FEQ == 3) {
+        // node_eq's k0 + t (k1 + t (k2 + t k3)) with k0 = drho + rho c0 and
+        // k_n = rho c_n, factored as drho + rho (c0 + t (c1 + t (c2 + t c3))):
+        // a reader forms two per-record coefficients (c2, c3 are constants)
+        // instead of four.  Every reader and the writer of a record use this
+        // same expression.
+        const float drho = m.a0, ux = m.a1, uy = m.a2;
+        const float rho = p.rho0 + drho;
+        const float usq = ux * ux + uy * uy;
+        const float c0 = -0.5f * p.s * usq;
+        const float c1 = p.s - 0.5f * p.s2 * usq;
+        const float c2 = 0.5f * p.s2;
+        const float c3 = (1.0f / 6.0f) * (p.s * p.s2);
+        const float t = udot<QID, I>(ux, uy);
+        return p.w[I] * (drho + rho * (c0 + t * (c1 + t * (c2 + t * c3))));
     } else {
         return horner_geq<QID, I>(node_eq<QID>(m.a0, m.a1, m.a2, p), p);
     }
