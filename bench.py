@@ -48,6 +48,8 @@ def tgv_populations(L, q, n, u0=0.02):
 ACCURACY_STEPS = 5000
 # fused launches timed one by one for the spread of the kernel's duration
 SPREAD_LAUNCHES = 200
+# back-to-back device copies of the FP64 lattice for the copy-rate ceiling
+COPY_LAUNCHES = 50
 
 
 def run_storage(L, q, f0, cm, storage, steps, warmup, accuracy_steps):
@@ -80,6 +82,24 @@ def run_storage(L, q, f0, cm, storage, steps, warmup, accuracy_steps):
     m = L.moments(acc.f_stream, stress=False)
     del acc
     return model, wall, kernel_ms, nlaunch, L.average_clock_mhz(c0, c1), (m.ux, m.uy)
+
+
+def copy_rate(L, f, copies=COPY_LAUNCHES):
+    """Rate of a plain device copy of the same bytes on the same GPU
+    (``lbm_field_copy``: 16-byte lanes, non-temporal stores), ``copies``
+    launches back to back timed by the host clock: the practical ceiling a
+    kernel that reads and writes every byte once is read against."""
+    g = f.copy()
+    for _ in range(3):
+        g.copyto_(f)
+    L.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(copies):
+        g.copyto_(f)
+    L.synchronize()
+    dt = (time.perf_counter() - t0) / copies
+    g.close()
+    return dt
 
 
 def launch_spread(L, model, launches=SPREAD_LAUNCHES):
@@ -544,6 +564,8 @@ def main():
     kavg64 = kms64 / max(nl64, 1)
     gbs64 = bytes64 / (kavg64 * 1e-3) / 1e9
     spread64 = launch_spread(L, m64)
+    copy_s = copy_rate(L, m64.f_stream)
+    copy_gbs = bytes64 / copy_s / 1e9
 
     result = {
         "metric": METRIC,
@@ -580,6 +602,11 @@ def main():
             "bytes_per_launch": bytes64,
             "shader_clock_mhz": _r(clk64),
             "spread": spread64,
+            "copy": {"gbs": round(copy_gbs, 1), "us_per_copy": round(copy_s * 1e6, 2),
+                     "frac_of_copy": round(gbs64 / copy_gbs, 4),
+                     "how": f"lbm_field_copy of the FP64 lattice ({bytes64 // 2} bytes, "
+                            f"read and written; 16-byte lanes, non-temporal stores), "
+                            f"{COPY_LAUNCHES} launches back to back, host clock"},
         },
     }
 
