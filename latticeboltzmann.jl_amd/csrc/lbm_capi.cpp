@@ -65,6 +65,10 @@ struct HostQuad {
     int cx[37], cy[37], opp[37];
     double w[37];
     double s = 0.0;
+    // the FP32 storages' weights and s (see tune_f32); f32_set = false: rounded
+    float w32[37] = {};
+    float s32 = 0.0f;
+    bool f32_set = false;
 };
 
 template <int QID>
@@ -82,7 +86,7 @@ void fill_tables(HostQuad &q) {
 
 // Weights and speed_of_sound_squared with the reference's expressions
 // (src/quadratures/D2Q{4,5,9,13,17,21,37}.jl).
-bool host_quad(int qid, HostQuad &q) {
+static bool host_quad_exact(int qid, HostQuad &q) {
     q = HostQuad{};
     q.id = qid;
     switch (qid) {
@@ -148,6 +152,130 @@ bool host_quad(int qid, HostQuad &q) {
     default:
         return false;
     }
+}
+
+// FP32 weights and s for the FP32-arithmetic storages.  Rounded one by one,
+// the discrete identities that make the equilibrium conserve mass and
+// momentum, sum_i w_i = 1 and s sum_i w_i c_ix^2 = 1, hold in FP32 only to
+// 1e-8 - 3e-8 on the sets with irrational weights (D2Q17: -1.3e-8 and
+// -2.8e-8).  The collision turns a momentum defect e into a push of
+// omega e rho u per step, which a uniformly accelerated flow integrates to a
+// relative velocity error of about omega e t / 2 (-3.5e-4 after 20,000 steps
+// at tau = 0.8 on D2Q17; -5.3e-4 measured).  Each group of equal weights and s
+// move here by at most K ulps from their rounding, to the combination that
+// minimises max(|sum w - 1|, |s sum w c_x^2 - 1|) (sums of floats times
+// integers, exact in double); it is kept when it at least halves the defect
+// of the plain rounding (D2Q13/17/21/37; D2Q4/5/9 keep the rounded values,
+// s = 3 stays exact on D2Q9).
+static void tune_f32(HostQuad &q) {
+    double gw[37];
+    int gn[37], gm[37], grp[37], ng = 0;
+    for (int i = 0; i < q.Q; ++i) {
+        grp[i] = -1;
+        if (q.w[i] == 0.0) continue;
+        int g = 0;
+        while (g < ng && gw[g] != q.w[i]) ++g;
+        if (g == ng) {
+            gw[ng] = q.w[i];
+            gn[ng] = gm[ng] = 0;
+            ++ng;
+        }
+        grp[i] = g;
+        gn[g] += 1;
+        gm[g] += q.cx[i] * q.cx[i];
+    }
+    const int K = ng <= 6 ? 3 : 2, span = 2 * K + 1, KS = 2;
+    // candidates: the rounding stepped -K .. K ulps (index K = the rounding)
+    float cand[37][7], scand[2 * KS + 1];
+    for (int g = 0; g < ng; ++g) {
+        const float r = static_cast<float>(gw[g]);
+        cand[g][K] = r;
+        for (int k = 1; k <= K; ++k) {
+            cand[g][K + k] = std::nextafter(cand[g][K + k - 1], 2.0f);
+            cand[g][K - k] = std::nextafter(cand[g][K - k + 1], 0.0f);
+        }
+    }
+    const float sr = static_cast<float>(q.s);
+    scand[KS] = sr;
+    for (int k = 1; k <= KS; ++k) {
+        scand[KS + k] = std::nextafter(scand[KS + k - 1], 1e30f);
+        scand[KS - k] = std::nextafter(scand[KS - k + 1], 0.0f);
+    }
+    auto defect = [&](const int *k, int ks, int &moves) {
+        double m = -1.0, e = 0.0;
+        moves = ks > KS ? ks - KS : KS - ks;
+        for (int g = 0; g < ng; ++g) {
+            m += gn[g] * (double)cand[g][k[g]];
+            e += gm[g] * (double)cand[g][k[g]];
+            moves += k[g] > K ? k[g] - K : K - k[g];
+        }
+        e = (double)scand[ks] * e - 1.0;
+        return std::fmax(std::fabs(m), std::fabs(e));
+    };
+    int k[37], best_k[37], moves = 0, best_moves = 0, best_ks = KS;
+    for (int g = 0; g < ng; ++g) k[g] = best_k[g] = K;
+    const double rounded = defect(k, KS, moves);
+    double best = rounded;
+    for (int ks = 0; ks <= 2 * KS; ++ks) {
+        for (int g = 0; g < ng; ++g) k[g] = 0;
+        while (true) {
+            const double d = defect(k, ks, moves);
+            if (d < best || (d == best && moves < best_moves)) {
+                best = d;
+                best_moves = moves;
+                best_ks = ks;
+                for (int g = 0; g < ng; ++g) best_k[g] = k[g];
+            }
+            int g = 0;
+            while (g < ng && ++k[g] == span) k[g++] = 0;
+            if (g == ng) break;
+        }
+    }
+    const bool use = best <= 0.5 * rounded;
+    for (int i = 0; i < 37; ++i)
+        q.w32[i] = i < q.Q ? (use && grp[i] >= 0 ? cand[grp[i]][best_k[grp[i]]]
+                                                 : static_cast<float>(q.w[i]))
+                           : 0.0f;
+    q.s32 = use ? scand[best_ks] : sr;
+    q.f32_set = true;
+}
+
+bool host_quad(int qid, HostQuad &q) {
+    struct Table {
+        HostQuad q[38];
+        bool ok[38] = {};
+        Table() {
+            for (int id : {4, 5, 9, 13, 17, 21, 37}) {
+                ok[id] = host_quad_exact(id, q[id]);
+                if (ok[id]) tune_f32(q[id]);
+            }
+        }
+    };
+    static const Table table;
+    if (qid < 0 || qid > 37 || !table.ok[qid]) {
+        q = HostQuad{};
+        return false;
+    }
+    q = table.q[qid];
+    return true;
+}
+
+// the quadrature constants of a launch (FP64 values; the FP32 storages take
+// w32 / s32)
+static void quad_consts(lbm::StepParams<double> &p, const HostQuad &q) {
+    for (int i = 0; i < 37; ++i) p.w[i] = i < q.Q ? q.w[i] : 0.0;
+    const double s = q.s;
+    p.s = s;
+    p.neg_s = -s;
+    p.s2 = s * s;
+    p.three_s = 3.0 * s;
+    p.six_s3 = 6.0 * ((s * s) * s);
+    p.three_s2 = 3.0 * (s * s);
+    p.s4 = pow4(s);
+    p.cs_inv = 1.0 / s;
+    for (int i = 0; i < 37; ++i) p.w32[i] = q.w32[i];
+    p.s32 = q.s32;
+    p.f32_set = q.f32_set ? 1 : 0;
 }
 
 int storage_bytes(int storage) {
@@ -241,16 +369,7 @@ lbm::StepParams<double> make_params(const lbm_model_s *m, const lbm_field_s *f, 
     p.nx = f->nx;
     p.ny = f->ny;
     p.plane = (long long)f->nx * (long long)f->ny;
-    for (int i = 0; i < 37; ++i) p.w[i] = i < q.Q ? q.w[i] : 0.0;
-    const double s = q.s;
-    p.s = s;
-    p.neg_s = -s;
-    p.s2 = s * s;
-    p.three_s = 3.0 * s;
-    p.six_s3 = 6.0 * ((s * s) * s);
-    p.three_s2 = 3.0 * (s * s);
-    p.s4 = pow4(s);
-    p.cs_inv = 1.0 / s;
+    quad_consts(p, q);
     if (m->coll == LBM_SRT) {
         p.omo = 1.0 - 1.0 / m->tau;
         p.om = 1.0 / m->tau;
@@ -269,7 +388,7 @@ lbm::StepParams<double> make_params(const lbm_model_s *m, const lbm_field_s *f, 
             const double t = (size_t)(n - 1) < m->taus.size() ? m->taus[n - 1] : 1.0;
             p.mrt_omo[n] = 1.0 - 1.0 / t;
             p.mrt_om[n] = 1.0 / t;
-            p.mrt_sn[n] = std::pow(s, (double)n);
+            p.mrt_sn[n] = std::pow(q.s, (double)n);
         }
         p.tau_force = m->taus.size() > 1 ? m->taus[1] : 1.0;
     }
@@ -553,16 +672,7 @@ static lbm::StepParams<double> field_params(const lbm_field_s *f) {
     p.nx = f->nx;
     p.ny = f->ny;
     p.plane = (long long)f->nx * f->ny;
-    for (int i = 0; i < 37; ++i) p.w[i] = i < q.Q ? q.w[i] : 0.0;
-    const double s = q.s;
-    p.s = s;
-    p.neg_s = -s;
-    p.s2 = s * s;
-    p.three_s = 3.0 * s;
-    p.six_s3 = 6.0 * ((s * s) * s);
-    p.three_s2 = 3.0 * (s * s);
-    p.s4 = pow4(s);
-    p.cs_inv = 1.0 / s;
+    quad_consts(p, q);
     p.rho0 = f->rho0;
     p.dscale = f->dscale;
     p.dscale_inv = 1.0 / f->dscale;
@@ -1481,16 +1591,7 @@ int batch_grid(const lbm_batch_s *b, int block) {
 lbm::StepParams<double> batch_params(const lbm_batch_s *b) {
     lbm::StepParams<double> p{};
     const HostQuad &q = b->q;
-    for (int i = 0; i < 37; ++i) p.w[i] = i < q.Q ? q.w[i] : 0.0;
-    const double s = q.s;
-    p.s = s;
-    p.neg_s = -s;
-    p.s2 = s * s;
-    p.three_s = 3.0 * s;
-    p.six_s3 = 6.0 * ((s * s) * s);
-    p.three_s2 = 3.0 * (s * s);
-    p.s4 = pow4(s);
-    p.cs_inv = 1.0 / s;
+    quad_consts(p, q);
     p.rho0 = b->rho0;
     p.dscale = p.dscale_inv = 1.0;
     return p;

@@ -45,6 +45,11 @@ struct StepParams {
     T cs_inv;                // 1/s
     // iterative initialization (FP64 only): nonlinear equilibrium term (nx, ny, Q)
     const double *mei;
+    // FP32 weights and s for the FP32-arithmetic storages (host side only;
+    // f32_set = 0: round w and s instead)
+    float w32[37];
+    float s32;
+    int f32_set;
 };
 
 // FP32 copy of the parameters for the FP32-arithmetic storages
@@ -53,9 +58,9 @@ inline StepParams<float> to_float_params(const StepParams<double> &d) {
     f.nx = d.nx;
     f.ny = d.ny;
     f.plane = d.plane;
-    for (int i = 0; i < 37; ++i) f.w[i] = static_cast<float>(d.w[i]);
-    f.s = static_cast<float>(d.s);
-    f.neg_s = static_cast<float>(d.neg_s);
+    for (int i = 0; i < 37; ++i) f.w[i] = d.f32_set ? d.w32[i] : static_cast<float>(d.w[i]);
+    f.s = d.f32_set ? d.s32 : static_cast<float>(d.s);
+    f.neg_s = -f.s;
     f.s2 = static_cast<float>(d.s2);
     f.three_s = static_cast<float>(d.three_s);
     f.six_s3 = static_cast<float>(d.six_s3);
