@@ -103,6 +103,15 @@ int lbm_clock_sample(uint64_t *out, double *tick_mhz);
 int lbm_quadrature_info(int quadrature, int *Q, double *cx, double *cy, double *w,
                         double *s, int *order, int *opp /* 0-based */);
 
+/* The FP32 weights (Q) and s = 1/c_s^2 that the FP32-arithmetic storages
+ * compute with.  No counterpart in the reference (whose arithmetic is FP64):
+ * on D2Q13/17/21/37 each weight group and s sit within a few ulps of their
+ * FP32 rounding, chosen so that sum w = 1 and s sum w c_x^2 = 1 hold in FP32
+ * to ~1e-9 (rounded one by one they hold to 1e-8 - 3e-8, which makes the
+ * FP32 collision gain or lose momentum in proportion to u); D2Q4/5/9 use the
+ * plain rounding. */
+int lbm_quadrature_f32(int quadrature, float *w, float *s);
+
 /* ---- population fields (device arrays) -------------------------------- */
 /* Array{Float64}(undef, nx, ny, Q) on the device (initial_conditions.jl:13).
  * rho0 is the rest density about which the *DEV storages store deviations. */

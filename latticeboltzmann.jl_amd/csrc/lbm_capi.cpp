@@ -602,6 +602,15 @@ int lbm_quadrature_info(int quadrature, int *Q, double *cx, double *cy, double *
     return 0;
 }
 
+int lbm_quadrature_f32(int quadrature, float *w, float *s) {
+    HostQuad q;
+    if (!host_quad(quadrature, q)) return fail(-1, "unknown quadrature D2Q%d", quadrature);
+    for (int i = 0; i < q.Q; ++i)
+        if (w) w[i] = q.w32[i];
+    if (s) *s = q.s32;
+    return 0;
+}
+
 // ---- fields -----------------------------------------------------------------
 int lbm_field_create(lbm_field *out, int quadrature, int nx, int ny, int storage, double rho0) {
     if (!out) return fail(-1, "lbm_field_create: null output handle");

@@ -23,7 +23,7 @@ NORTH, EAST, SOUTH, WEST = 0, 1, 2, 3
 # every entry point declared in include/lbm_hip.h
 EXPORTS = [
     "lbm_last_error", "lbm_version", "lbm_device_count", "lbm_synchronize",
-    "lbm_clock_sample", "lbm_quadrature_info",
+    "lbm_clock_sample", "lbm_quadrature_info", "lbm_quadrature_f32",
     "lbm_field_create", "lbm_field_destroy", "lbm_field_info", "lbm_field_upload",
     "lbm_field_download", "lbm_field_copy", "lbm_initialize_analytical",
     "lbm_model_create", "lbm_model_destroy", "lbm_model_set_srt", "lbm_model_set_trt",
@@ -97,6 +97,7 @@ def lib() -> ctypes.CDLL:
     L.lbm_synchronize.restype = i
     L.lbm_clock_sample.argtypes = [ctypes.POINTER(ctypes.c_uint64), dp]
     L.lbm_quadrature_info.argtypes = [i, ip, dp, dp, dp, dp, ip, ip]
+    L.lbm_quadrature_f32.argtypes = [i, vp, vp]
     L.lbm_field_create.argtypes = [ctypes.POINTER(vp), i, i, i, i, d]
     L.lbm_field_destroy.argtypes = [vp]
     L.lbm_field_info.argtypes = [vp, ip, ip, ip, ip, ip]
