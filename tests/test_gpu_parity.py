@@ -1,7 +1,7 @@
 """GPU parity: the gfx950 kernels (through the C ABI) against the CPU oracle.
 
 Bar: the plain FP64 path is bit-exact against the oracle (same evaluation
-order, no FMA contraction).  The north-star tolerance for the FP64 path is
+order, no FMA contraction).  The required tolerance for the FP64 path is
 1e-12 relative per population after N steps; asserting equality is stricter.
 """
 import numpy as np
