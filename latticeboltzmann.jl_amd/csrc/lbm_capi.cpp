@@ -240,6 +240,8 @@ static void tune_f32(HostQuad &q) {
     q.f32_set = true;
 }
 
+// the constants of every quadrature, built once per process on first use
+// (the FP32 search takes about 40 ms in all)
 bool host_quad(int qid, HostQuad &q) {
     struct Table {
         HostQuad q[38];
