@@ -456,17 +456,16 @@ void auto_launch(const lbm_model_s *m, const lbm_field_s *f, int &block, int &ro
     //   256 threads: 166-169 us against 181 with 4 and 189 with 8);
     // * D2Q4/5/9 on odd widths (one column per lane): 4 rows (4095^2: 224 us
     //   against 227 with 2 and 245 with 8);
-    // * D2Q13: 8 rows; D2Q17: 16 rows (8192 x 2048 after the third-order
-    //   record factoring, 256 threads: 366 us against 372 with 8 and 407 with
-    //   4; 128 threads the same, 64 threads 527);
-    // * D2Q21: 2 rows (as D2Q37 before its coefficient window: 887-892 us
-    //   against 1015-1035 with 4);
-    // * D2Q37, coefficient window: 8 rows (8192 x 2048: 747 us against 788 with
-    //   4, 875 with 2, 737 with 16).
+    // * 8192 x 2048, 256 threads (D2Q17/21 after the third-order record
+    //   factoring):
+    //   - D2Q13: 8 rows (253 us against 259 with 16 and 263 with 4);
+    //   - D2Q17: 16 rows (366 us against 372 with 8 and 407 with 4; 128
+    //     threads the same, 64 threads 527);
+    //   - D2Q21: 8 rows (543 us against 544 with 16, 559 with 2, 575 with 4);
+    //   - D2Q37, coefficient window: 8 rows (747 us against 788 with 4, 875
+    //     with 2, 737 with 16).
     if (vec <= 0 && rows_mode(f->storage) && m->coll != LBM_MRT) {
-        if (f->Q == 37) vec = 8;
-        else if (f->Q > 17) vec = 2;
-        else if (f->Q == 17) vec = 16;
+        if (f->Q == 17) vec = 16;
         else if (f->Q > 9) vec = 8;
         else vec = f->nx % 2 == 0 && f->nx > 4 ? 2 : 4;
     }
